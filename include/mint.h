@@ -1,0 +1,213 @@
+/*
+ * mint.h — C ABI of the MI355X (gfx950) Mercurius step library, libmint.so.
+ *
+ * Units: G = 1, masses in solar masses, lengths in AU, time in years.
+ * All state is FP64.  A "system" is one independent sweep member: a central
+ * mass M (the Sun) at the heliocentric origin, n_massive massive bodies and a
+ * variable number of massless test particles (TPs).
+ *
+ * The reference has no plugin or FFI layer; its boundary is the kernel
+ *   mercurius_solver(ω[], M[], e[], a[], i[], Ω[], m[], pos_out[], N_m, batch, dt)
+ *   (src/sim.cuh:615-625), launched by cudaLaunchKernelEx from main
+ *   (src/mercurius.cu:45-65,184-198),
+ * with host types Body/Sweep/Sim (src/simutils.cuh:19-87).  Each entry point
+ * below cites the part of that interface it replaces.
+ *
+ * Conventions:
+ *   - every call returns 0 on success and a negative MINT_E* code on failure;
+ *     mint_last_error() returns the message of the last failure on a context;
+ *   - the context owns all device memory; the caller owns host buffers, which
+ *     are copied in and out and never retained after a call returns;
+ *   - a context is used by one host thread at a time and issues all work on
+ *     its own HIP stream; calls return after that work has finished.
+ */
+#ifndef MINT_H
+#define MINT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MINT_ABI_VERSION 1
+
+/* error codes */
+#define MINT_OK 0
+#define MINT_EINVAL (-1)   /* bad argument or shape */
+#define MINT_EHIP (-2)     /* HIP runtime failure */
+#define MINT_ENOMEM (-3)   /* device or host allocation failed */
+#define MINT_ESTATE (-4)   /* call out of order (e.g. step before upload) */
+#define MINT_EIO (-5)      /* file or parse error (config, weights, cache) */
+
+/* Drift forms (how the TP population is scheduled on the GPU). */
+#define MINT_FORM_PLAIN 0       /* one launch per step, a few systems per launch, in-place BS */
+#define MINT_FORM_FLAT 1        /* one launch per step over all systems, in-place BS */
+#define MINT_FORM_BATCHED 2     /* multi-step register-resident kernel + BS encounter queue, hipGraph */
+
+/* Kepler-solver starting guesses for the TP drift. */
+#define MINT_GUESS_CRUDE 0      /* s0 = dt / r0 */
+#define MINT_GUESS_SERIES 1     /* third-order series in dt */
+#define MINT_GUESS_LEARNED 2    /* FP32 MLP correction on top of the series guess */
+
+/* Per-system status bits (mint_status). */
+#define MINT_ST_NAN 1u              /* a non-finite value appeared in TP or massive state */
+#define MINT_ST_NEWTON 2u           /* a Kepler solve hit the iteration cap */
+#define MINT_ST_BS_DEPTH 4u         /* a Bulirsch-Stoer step hit its subdivision cap */
+#define MINT_ST_MASSIVE_ENC 8u      /* massive bodies came within their changeover radius */
+
+/*
+ * Integration parameters.  Replaces the compile-time constants of
+ * src/constants.cuh:4-13 and the hard-coded dt of src/mercurius.cu:42-43.
+ * mint_config_default() fills the defaults listed on each field.
+ */
+typedef struct mint_config {
+  double dt;             /* step length [yr]; default 0.8219 (src/mercurius.cu:43) */
+  double central_mass;   /* M; default 1.0 (src/sim.cuh:401) */
+  double n_hill;         /* changeover radius r_c = n_hill * Hill radius; default 3 */
+  double softening2;     /* eps^2 of the pair force; default 1e-8 (src/constants.cuh:8) */
+  double bs_tol;         /* Bulirsch-Stoer tolerance; default 1e-12 (src/constants.cuh:6) */
+  int32_t form;          /* MINT_FORM_*; default MINT_FORM_BATCHED */
+  int32_t kepler_guess;  /* MINT_GUESS_*; default MINT_GUESS_SERIES */
+  int32_t systems_per_launch; /* plain form only; default 4 (src/constants.cuh:12) */
+  int32_t block_size;    /* workgroup size; 0 = autotuned (batched) or 256 (per-step forms) */
+  int32_t tps_per_lane;  /* batched: TPs per lane (1, 2 or 4); 0 = autotuned */
+  int32_t chunk_steps;   /* batched: steps per register-resident launch; 0 = autotuned */
+  int32_t use_scorer;    /* batched: pre-bucket likely encounters with the learned scorer */
+  int32_t use_graph;     /* batched: replay chunk launches from a captured hipGraph; default 1 */
+  int32_t device;        /* HIP device ordinal; default 0 (the -d flag, src/simutils.cuh:170-173) */
+  int32_t collect_stats; /* bit 0: count work into mint_stats; bit 1: per-kernel HIP-event
+                            timing into mint_timing (launches without the graph); default 0 */
+} mint_config;
+
+/*
+ * Host-side description of a batch of systems (the CSR layout that replaces
+ * the reference's [sweep][body] element arrays, src/sim.cuh:652 and
+ * src/mercurius.cu:73-164).  All pointers are host pointers.
+ *   tp_offset[S+1]   prefix sums of TP counts; TPs of system s are
+ *                    tp_offset[s] .. tp_offset[s+1]-1 in the SoA arrays
+ *   m[S*Nm]          massive-body masses (> 1e-12, src/mercurius.cu:9)
+ *   mx, mv[S*Nm*3]   heliocentric positions / velocities of massive bodies
+ *   tx..tvz[Ntp]     heliocentric TP positions / velocities, SoA
+ *   rc[S*Nm]         changeover radii; NULL on upload = computed from n_hill
+ */
+typedef struct mint_state {
+  int32_t n_systems;
+  int32_t n_massive;
+  const int64_t *tp_offset;
+  double *m;
+  double *mx;
+  double *mv;
+  double *tx, *ty, *tz, *tvx, *tvy, *tvz;
+  double *rc;
+} mint_state;
+
+/* Work counters (collect_stats = 1). */
+typedef struct mint_stats {
+  int64_t tp_steps;        /* TP steps taken */
+  int64_t kepler_solves;   /* TP Kepler drifts */
+  int64_t newton_iters;    /* Newton iterations over those drifts */
+  int64_t bs_steps;        /* TP steps drifted by Bulirsch-Stoer */
+  int64_t bs_rhs_evals;    /* right-hand-side evaluations inside BS */
+  int64_t deferred;        /* batched: TP-steps handed to the encounter queue */
+  int64_t prebucketed;     /* batched: TPs routed to the queue by the scorer */
+  int64_t idle_lane_steps; /* batched: lane-steps left idle in the main kernel */
+} mint_stats;
+
+/* Per-kernel timings from HIP events on the context stream (ms, launches). */
+typedef struct mint_timing {
+  double ms_ephemeris;  int64_t n_ephemeris;
+  double ms_tp_main;    int64_t n_tp_main;
+  double ms_encounter;  int64_t n_encounter;
+  double ms_total;
+} mint_timing;
+
+int32_t mint_abi_version(void);
+void mint_config_default(mint_config *cfg);
+
+/* Create a context on cfg->device (replaces cudaSetDevice + the cluster launch
+ * set-up, src/mercurius.cu:36,45-70). */
+int mint_create(const mint_config *cfg, void **ctx);
+void mint_destroy(void *ctx);
+const char *mint_last_error(void *ctx);
+
+/* Copy a batch to the device, converting velocities to democratic-heliocentric
+ * form (replaces the H2D copies src/mercurius.cu:73-164 and the setup
+ * conversion src/sim.cuh:665-677).  If st->rc is NULL the changeover radii are
+ * computed from the uploaded state (mint_hill_radii) and stored. */
+int mint_upload(void *ctx, const mint_state *st);
+
+/* Advance every system by nsteps Mercurius steps of cfg->dt (replaces the
+ * per-batch launch loop src/mercurius.cu:179-198 and the step loop body
+ * src/sim.cuh:679-742).  Returns after the device work has finished. */
+int mint_step(void *ctx, int64_t nsteps);
+
+/* Copy the state back, converting velocities to heliocentric form (replaces
+ * the teardown src/sim.cuh:744-765 and the D2H copies src/mercurius.cu:203-243).
+ * Only the non-NULL arrays of st are written; shapes must match the upload. */
+int mint_download(void *ctx, mint_state *st);
+
+/* Total energy of each system's massive bodies (democratic-heliocentric
+ * Hamiltonian; the reference has no energy diagnostic). E[S]. */
+int mint_energy(void *ctx, double *E);
+
+/* OR of status bits seen per system since upload. status[S]. */
+int mint_status(void *ctx, uint32_t *status);
+
+int mint_get_stats(void *ctx, mint_stats *out);
+int mint_get_timing(void *ctx, mint_timing *out);
+int mint_reset_counters(void *ctx);
+
+/* Change kepler_guess / use_scorer / form between calls without re-uploading. */
+int mint_set_mode(void *ctx, int32_t form, int32_t kepler_guess, int32_t use_scorer);
+
+/* Changeover radii r_c = n_hill * a * cbrt(m / (3M)) from heliocentric states
+ * (replaces fetch_r_crit, src/sim.cuh:133-164, evaluated once instead of per
+ * pair).  a is the osculating semi-major axis; unbound orbits use r. */
+int mint_hill_radii(int64_t n, double central_mass, double n_hill, const double *m,
+                    const double *x, const double *v, double *rc);
+
+/* Osculating elements (i, Ω, ω, M, e, a) -> heliocentric Cartesian for
+ * n bodies, mu = G M (replaces cartesian_from_elements, src/sim.cuh:188-239,
+ * at the configuration edge).  elements are SoA arrays of length n. */
+int mint_elements_to_cartesian(int64_t n, double mu, const double *inc, const double *node,
+                               const double *argp, const double *mean_anom, const double *ecc,
+                               const double *sma, double *x, double *v);
+/* Inverse (replaces elements_from_cartesian, src/sim.cuh:241-324, for output). */
+int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const double *v,
+                               double *inc, double *node, double *argp, double *mean_anom,
+                               double *ecc, double *sma);
+
+/* Autotuner (batched form): time the candidate launch shapes on a sample of the
+ * uploaded batch, keep the fastest, and store it in cache_path (NULL = no file).
+ * The state on the device is left unchanged.  best[3] = {block, tps/lane, chunk}. */
+int mint_autotune(void *ctx, const char *cache_path, int32_t *best);
+
+/* Learned parts (FP32).  Weights live in the context.
+ * Kepler warm-start MLP: 4 -> 32 -> 32 -> 1, tanh.  Scorer: 6 -> 16 -> 1, tanh/logistic.
+ * mint_trace_*: run nsteps with trace logging into a device ring buffer of
+ * capacity `cap` samples, then copy samples out (features then target). */
+int mint_trace_kepler(void *ctx, int64_t nsteps, int64_t cap, float *feat /*cap*4*/,
+                      float *target /*cap*/, int64_t *n_out);
+int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat /*cap*6*/,
+                         float *label /*cap*/, int64_t *n_out);
+/* Train on the GPU with the library's FP32 forward/backward/Adam kernels.
+ * which = 0 Kepler MLP, 1 scorer.  loss_out[epochs] receives mean losses. */
+int mint_train(void *ctx, int32_t which, int64_t n, const float *feat, const float *target,
+               int32_t epochs, int32_t batch, float lr, uint64_t seed, float *loss_out);
+/* Forward / backward of one network on n samples (device kernels; for tests).
+ * grad[nparams] receives the gradient of 0.5*mean((y - target)^2) (which = 0)
+ * or of mean BCE-with-logits (which = 1). */
+int mint_nn_forward(void *ctx, int32_t which, int64_t n, const float *feat, float *out);
+int mint_nn_loss_grad(void *ctx, int32_t which, int64_t n, const float *feat, const float *target,
+                      float *loss, float *grad);
+int mint_nn_nparams(int32_t which);
+/* Pre-bucket a TP when the scorer's largest pair logit exceeds t (default 0). */
+int mint_set_score_threshold(void *ctx, float t);
+int mint_nn_get_weights(void *ctx, int32_t which, float *w);
+int mint_nn_set_weights(void *ctx, int32_t which, const float *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MINT_H */

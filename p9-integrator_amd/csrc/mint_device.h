@@ -1,0 +1,608 @@
+// mint_device.h — FP64 device numerics of the Mercurius step for gfx950.
+//
+// Every function here is plain IEEE arithmetic: +, -, *, /, sqrt and explicit
+// fma, compiled with -ffp-contract=off, so the same operation sequence gives
+// the same bits on the host.  No transcendental function is called on the
+// step path (the Stumpff functions are a fixed-length series with argument
+// quartering), which is what lets a host FP64 restatement match bit for bit.
+//
+// Reference functions each piece replaces are cited as file:line of
+// at2005/p9-integrator.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mint {
+
+constexpr int KMAX = 8;            // BS rows, substeps N_k = 2^(k+1) = 2..256
+constexpr int BS_MAXDEPTH = 24;    // interval halvings before giving up
+constexpr int NEWTON_MAXIT = 40;   // Kepler Newton iteration cap
+
+constexpr uint32_t ST_NAN = 1u, ST_NEWTON = 2u, ST_BS_DEPTH = 4u, ST_MASSIVE_ENC = 8u;
+
+// Per-system parameter block (doubles), NM massive bodies:
+//   [0,NM)           m_j
+//   [NM,2NM)         0.1 r_c,j            TP-body changeover, r_c = r_c,j
+//   [2NM,3NM)        1/(0.9 r_c,j)
+//   [3NM,4NM)        r_c,j^2
+//   [4NM, 4NM+NM^2)          pair 0.1 r_c   massive-massive, r_c = max(r_c,j, r_c,k)
+//   [4NM+NM^2, 4NM+2NM^2)    pair 1/(0.9 r_c)
+//   [4NM+2NM^2, 4NM+3NM^2)   pair r_c^2
+__host__ __device__ constexpr int prm_size(int nm) { return ((4 * nm + 3 * nm * nm) + 1) & ~1; }
+__host__ __device__ constexpr int P_M(int) { return 0; }
+__host__ __device__ constexpr int P_RCLO(int nm) { return nm; }
+__host__ __device__ constexpr int P_RCWINV(int nm) { return 2 * nm; }
+__host__ __device__ constexpr int P_RC2(int nm) { return 3 * nm; }
+__host__ __device__ constexpr int P_PRCLO(int nm) { return 4 * nm; }
+__host__ __device__ constexpr int P_PRCWINV(int nm) { return 4 * nm + nm * nm; }
+__host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; }
+
+// Ephemeris record of one step (doubles): what every TP phase of that step
+// reads from the massive subsystem (the per-phase reads of block 0's shared
+// memory in src/sim.cuh:682-741).
+//   [0,3)  Q1 = (sum m_j v_j)/M after the first kick      (jump 1)
+//   [3,6)  Q2 = (sum m_j v_j)/M after the drift            (jump 2)
+//   [6, 6+3NM)       Y: massive positions after jump 1     (encounter test, BS)
+//   [6+3NM, 6+6NM)   W: massive velocities at drift start  (BS perturber drift)
+//   [6+6NM, 6+9NM)   Xe: massive positions after jump 2    (closing kick)
+__host__ __device__ constexpr int rec_size(int nm) { return ((6 + 9 * nm) + 1) & ~1; }
+__host__ __device__ constexpr int R_Q1(int) { return 0; }
+__host__ __device__ constexpr int R_Q2(int) { return 3; }
+__host__ __device__ constexpr int R_Y(int) { return 6; }
+__host__ __device__ constexpr int R_W(int nm) { return 6 + 3 * nm; }
+__host__ __device__ constexpr int R_XE(int nm) { return 6 + 6 * nm; }
+
+struct StepConst {
+  double h;     // step
+  double hh;    // h/2
+  double mu;    // G*M
+  double minv;  // unused slot kept for alignment of the launch argument block
+  double eps2;  // pair softening
+  double tol;   // BS tolerance
+};
+
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by,
+                                       double bz) {
+  return fma(ax, bx, fma(ay, by, az * bz));
+}
+
+// Stumpff functions c0..c3(z) (Danby 1988 §6.9; Stiefel & Scheifele 1971):
+// quarter z until |z| <= 0.1, sum c2 and c3 as 7-term series, then undo the
+// quartering with c0(4z)=2c0^2-1, c1(4z)=c0c1, c2(4z)=c1^2/2,
+// c3(4z)=(c2+c0c3)/4.  Replaces the element round trip and the fixed
+// 8-iteration Kepler-equation solve of src/sim.cuh:90-126,188-324.
+__device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3) {
+  int n = 0;
+  while (fabs(z) > 0.1 && n < 64) {
+    z *= 0.25;
+    ++n;
+  }
+  double t2 = 1.1470745597729725e-11;
+  t2 = fma(t2, z, -2.08767569878681e-09);
+  t2 = fma(t2, z, 2.755731922398589e-07);
+  t2 = fma(t2, z, -2.48015873015873e-05);
+  t2 = fma(t2, z, 0.001388888888888889);
+  t2 = fma(t2, z, -0.041666666666666664);
+  t2 = fma(t2, z, 0.5);
+  double t3 = 7.647163731819816e-13;
+  t3 = fma(t3, z, -1.6059043836821613e-10);
+  t3 = fma(t3, z, 2.505210838544172e-08);
+  t3 = fma(t3, z, -2.7557319223985893e-06);
+  t3 = fma(t3, z, 0.0001984126984126984);
+  t3 = fma(t3, z, -0.008333333333333333);
+  t3 = fma(t3, z, 0.16666666666666666);
+  c2 = t2;
+  c3 = t3;
+  c1 = fma(-z, c3, 1.0);
+  c0 = fma(-z, c2, 1.0);
+  for (; n > 0; --n) {
+    const double n3 = fma(c0, c3, c2) * 0.25;
+    const double n2 = (c1 * c1) * 0.5;
+    const double n1 = c0 * c1;
+    const double n0 = fma(2.0 * c0, c0, -1.0);
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+}
+
+// Quantities of the universal-variable Kepler problem at the drift start.
+struct KepStart {
+  double r0, eta0, beta, zeta0;
+};
+
+__device__ __forceinline__ KepStart kep_start(double x, double y, double z, double vx, double vy,
+                                              double vz, double mu) {
+  KepStart k;
+  k.r0 = sqrt(dot3(x, y, z, x, y, z));
+  const double v2 = dot3(vx, vy, vz, vx, vy, vz);
+  k.eta0 = dot3(x, y, z, vx, vy, vz);
+  k.beta = (2.0 * mu) / k.r0 - v2;
+  k.zeta0 = fma(-k.beta, k.r0, mu);
+  return k;
+}
+
+// Starting guesses for the universal variable s.
+__device__ __forceinline__ double guess_crude(const KepStart &k, double dt) { return dt / k.r0; }
+// Third-order series inversion of dt = r0 s + eta0 s^2/2 + zeta0 s^3/6 + O(s^4).
+__device__ __forceinline__ double guess_series(const KepStart &k, double dt) {
+  const double u = dt / k.r0;
+  const double a = -(k.eta0 / (2.0 * k.r0));
+  const double b = (k.eta0 * k.eta0) / (2.0 * (k.r0 * k.r0)) - k.zeta0 / (6.0 * k.r0);
+  return u * fma(u, fma(u, b, a), 1.0);
+}
+
+// Universal-variable Kepler drift by dt about mu (Danby 1988 §6.9; the
+// G-function form of Mikkola & Innanen 1999).  Newton on
+//   F(s) = r0 s + eta0 G2 + zeta0 G3 - dt,  F' = r(s) = r0 + eta0 G1 + zeta0 G2,
+// stopping when the predicted post-update error |F''| ds^2 / (2 F') is below
+// 2^-52 |s| and |ds| <= 2e-6 |s|; the G-functions are then carried to s - ds by
+// a second-order Taylor step (G_k' = G_{k-1}) instead of a fresh evaluation.
+// Returns the Newton iteration count; ok=false if the cap was hit.
+struct KepSol {
+  double G0, G1, G2, G3;  // G-functions at s
+  double ds;              // last Newton correction; the root is s - ds
+  double s;
+  int it;
+  bool ok;
+};
+
+__device__ __forceinline__ KepSol kepler_solve(const KepStart &k, double s, double dt) {
+  KepSol o;
+  o.it = 0;
+  o.ok = false;
+  while (true) {
+    ++o.it;
+    double c0, c1, c2, c3;
+    const double ss = s * s;
+    stumpff(k.beta * ss, c0, c1, c2, c3);
+    o.G0 = c0;
+    o.G1 = s * c1;
+    o.G2 = ss * c2;
+    o.G3 = ss * (s * c3);
+    const double F = fma(k.r0, s, fma(k.eta0, o.G2, fma(k.zeta0, o.G3, -dt)));
+    const double rp = fma(k.eta0, o.G1, fma(k.zeta0, o.G2, k.r0));
+    const double rpp = fma(k.eta0, o.G0, k.zeta0 * o.G1);
+    o.ds = F / rp;
+    const double as = fabs(s);
+    if (fabs(o.ds) <= 2e-6 * as && fabs(rpp) * (o.ds * o.ds) <= 4.4e-16 * (fabs(rp) * as)) {
+      o.ok = true;
+      break;
+    }
+    if (o.it >= NEWTON_MAXIT) break;
+    s -= o.ds;
+  }
+  o.s = s;
+  return o;
+}
+
+__device__ __forceinline__ int kepler_drift_from(const KepStart &k, double s, double &x, double &y,
+                                                 double &z, double &vx, double &vy, double &vz,
+                                                 double mu, double dt, bool &ok) {
+  const KepSol o = kepler_solve(k, s, dt);
+  ok = o.ok;
+  const double hs = -o.ds;
+  const double hh2 = 0.5 * (hs * hs);
+  const double t0 = o.G0 * hh2;
+  const double t1 = o.G1 * hh2;
+  const double G1n = o.G1 + fma(o.G0, hs, -(k.beta * t1));
+  const double G2n = o.G2 + fma(o.G1, hs, t0);
+  const double G3n = o.G3 + fma(o.G2, hs, t1);
+  const double r = fma(k.eta0, G1n, fma(k.zeta0, G2n, k.r0));
+  const double fm1 = -((mu * G2n) / k.r0);
+  const double g = dt - mu * G3n;
+  const double fdot = -((mu * G1n) / (k.r0 * r));
+  const double gm1 = -((mu * G2n) / r);
+  const double nx = x + fma(fm1, x, g * vx);
+  const double ny = y + fma(fm1, y, g * vy);
+  const double nz = z + fma(fm1, z, g * vz);
+  const double nvx = vx + fma(fdot, x, gm1 * vx);
+  const double nvy = vy + fma(fdot, y, gm1 * vy);
+  const double nvz = vz + fma(fdot, z, gm1 * vz);
+  x = nx;
+  y = ny;
+  z = nz;
+  vx = nvx;
+  vy = nvy;
+  vz = nvz;
+  return o.it;
+}
+
+// Kepler drift with the series guess (massive bodies, BS perturbers).
+__device__ __forceinline__ int kepler_drift_series(double &x, double &y, double &z, double &vx,
+                                                   double &vy, double &vz, double mu, double dt,
+                                                   bool &ok) {
+  const KepStart k = kep_start(x, y, z, vx, vy, vz, mu);
+  return kepler_drift_from(k, guess_series(k, dt), x, y, z, vx, vy, vz, mu, dt, ok);
+}
+
+// Changeover-weighted pair acceleration factor (src/sim.cuh:166-186,327-382):
+// y = (r - 0.1 r_c)/(0.9 r_c); K = 0 (y<=0), y^2/(2y^2-2y+1) (0<y<1), 1 (y>=1).
+// The kick weight is K * m / ((r^2 + eps^2) r), folded into one division.
+__device__ __forceinline__ double kick_weight(double r2, double eps2, double m, double rclo,
+                                              double rcwinv) {
+  const double r = sqrt(r2);
+  const double D = (r2 + eps2) * r;
+  const double yv = (r - rclo) * rcwinv;
+  double num = m, den = D;
+  if (yv <= 0.0) {
+    num = 0.0;
+  } else if (yv < 1.0) {
+    const double y2 = yv * yv;
+    num = m * y2;
+    den = D * fma(2.0, y2, fma(-2.0, yv, 1.0));
+  }
+  return num / den;
+}
+
+// Close-encounter weight (1 - K) m / ((r^2 + eps^2) r) = (1-y)^2/(2y^2-2y+1) ...
+// zero for y >= 1 (replaces the double-counting weight of src/sim.cuh:357-361).
+__device__ __forceinline__ double close_weight(double r2, double eps2, double m, double rclo,
+                                               double rcwinv) {
+  const double r = sqrt(r2);
+  const double yv = (r - rclo) * rcwinv;
+  if (yv >= 1.0) return 0.0;
+  const double D = (r2 + eps2) * r;
+  double num = m, den = D;
+  if (yv > 0.0) {
+    const double om = 1.0 - yv;
+    num = m * (om * om);
+    den = D * fma(2.0, yv * yv, fma(-2.0, yv, 1.0));
+  }
+  return num / den;
+}
+
+// TP kick acceleration from the NM massive bodies at positions X (body-major
+// xyz).  Replaces body_interaction_kick for a massless body (src/sim.cuh:327-382).
+template <int NM>
+__device__ __forceinline__ void accel_tp(double x, double y, double z,
+                                         const double *__restrict__ X,
+                                         const double *__restrict__ prm, double eps2,
+                                         double &ax, double &ay, double &az) {
+  ax = 0.0;
+  ay = 0.0;
+  az = 0.0;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    const double dx = X[3 * j + 0] - x;
+    const double dy = X[3 * j + 1] - y;
+    const double dz = X[3 * j + 2] - z;
+    const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double w = kick_weight(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j],
+                                 prm[P_RCWINV(NM) + j]);
+    ax = fma(w, dx, ax);
+    ay = fma(w, dy, ay);
+    az = fma(w, dz, az);
+  }
+}
+
+// The exact distance check: is the TP within r_c,j of any massive body?
+// (close_encounter_p, src/sim.cuh:589-613, on squared distances.)
+template <int NM>
+__device__ __forceinline__ bool encounter_tp(double x, double y, double z,
+                                             const double *__restrict__ Y,
+                                             const double *__restrict__ prm) {
+  bool f = false;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    const double dx = Y[3 * j + 0] - x;
+    const double dy = Y[3 * j + 1] - y;
+    const double dz = Y[3 * j + 2] - z;
+    const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+    f = f || (r2 < prm[P_RC2(NM) + j]);
+  }
+  return f;
+}
+
+// Right-hand side of the close-encounter problem of one TP at time tau into
+// the drift: Kepler force of the (unsoftened) Sun plus (1-K)-weighted pulls of
+// the massive bodies, each on its own Kepler orbit from (Y_j, W_j).
+// (update_my_velocity_total, src/sim.cuh:410-443, with moving perturbers.)
+template <int NM>
+__device__ __forceinline__ void rhs_tp(double tau, const double *yv, const double *__restrict__ Y,
+                                       const double *__restrict__ W,
+                                       const double *__restrict__ prm, double mu, double eps2,
+                                       double *f) {
+  const double x = yv[0], y = yv[1], z = yv[2];
+  f[0] = yv[3];
+  f[1] = yv[4];
+  f[2] = yv[5];
+  const double r2 = dot3(x, y, z, x, y, z);
+  const double r = sqrt(r2);
+  const double w0 = mu / (r2 * r);
+  double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    double px = Y[3 * j + 0], py = Y[3 * j + 1], pz = Y[3 * j + 2];
+    if (tau != 0.0) {
+      double qx = W[3 * j + 0], qy = W[3 * j + 1], qz = W[3 * j + 2];
+      bool ok;
+      kepler_drift_series(px, py, pz, qx, qy, qz, mu, tau, ok);
+    }
+    const double dx = px - x, dy = py - y, dz = pz - z;
+    const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double w = close_weight(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j],
+                                  prm[P_RCWINV(NM) + j]);
+    ax = fma(w, dx, ax);
+    ay = fma(w, dy, ay);
+    az = fma(w, dz, az);
+  }
+  f[3] = ax;
+  f[4] = ay;
+  f[5] = az;
+}
+
+// Gragg's modified midpoint rule with N (even) substeps over [t0, t0+H]
+// (Hairer, Norsett & Wanner, Solving ODEs I, §II.9), on y = (x, v).
+// Replaces modified_midpoint (src/sim.cuh:445-500), whose half-updated
+// velocity does not have an even error expansion.
+template <int NM>
+__device__ __noinline__ void gragg_tp(const double *y0, const double *f0, double t0, double H,
+                                      int N, const double *__restrict__ Y,
+                                      const double *__restrict__ W,
+                                      const double *__restrict__ prm, double mu, double eps2,
+                                      double *out, int &nrhs) {
+  const double hs = H / (double)N;
+  const double h2 = 2.0 * hs;
+  double z0[6], z1[6], f[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    z0[i] = y0[i];
+    z1[i] = fma(hs, f0[i], y0[i]);
+  }
+  for (int m = 1; m < N; ++m) {
+    rhs_tp<NM>(t0 + (double)m * hs, z1, Y, W, prm, mu, eps2, f);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double z2 = fma(h2, f[i], z0[i]);
+      z0[i] = z1[i];
+      z1[i] = z2;
+    }
+  }
+  rhs_tp<NM>(t0 + H, z1, Y, W, prm, mu, eps2, f);
+  nrhs += N;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
+}
+
+__device__ __forceinline__ double bs_fac(int j) {
+  // 4^j - 1 for the doubling sequence (Richardson factor of src/sim.cuh:522-544)
+  return (double)((1ll << (2 * j)) - 1);
+}
+
+// Bulirsch-Stoer drift of one TP over the step h: Gragg rows N = 2,4,...,256,
+// Aitken-Neville extrapolation in H^2, error |T_kk - T_k,k-1| scaled by |x0|,
+// |v0| against tol; on failure the interval is halved.  Replaces the fixed
+// 3-row richardson_extrapolation of src/sim.cuh:502-551.  Returns status bits.
+template <int NM>
+__device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
+                                             const double *__restrict__ W,
+                                             const double *__restrict__ prm, double mu,
+                                             double eps2, double tol, int &nrhs) {
+  uint32_t st = 0;
+  double t = 0.0, H = h;
+  int depth = 0;
+  while (t < h) {
+    if (H > h - t) H = h - t;
+    double f0[6];
+    rhs_tp<NM>(t, yv, Y, W, prm, mu, eps2, f0);
+    nrhs += 1;
+    const double xs = sqrt(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
+    const double vs = sqrt(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
+    double R[KMAX][6];
+    double cur[6];
+    bool conv = false;
+    for (int k = 0; k < KMAX; ++k) {
+      gragg_tp<NM>(yv, f0, t, H, 2 << k, Y, W, prm, mu, eps2, cur, nrhs);
+      for (int j = 1; j <= k; ++j) {
+        const double fac = bs_fac(j);
+        for (int i = 0; i < 6; ++i) {
+          const double prev = R[j - 1][i];
+          R[j - 1][i] = cur[i];
+          cur[i] = cur[i] + (cur[i] - prev) / fac;
+        }
+      }
+      for (int i = 0; i < 6; ++i) R[k][i] = cur[i];
+      if (k >= 1) {
+        double err = 0.0;
+        for (int i = 0; i < 6; ++i) {
+          const double e = fabs(cur[i] - R[k - 1][i]) / (i < 3 ? xs : vs);
+          err = e > err ? e : err;
+        }
+        if (err <= tol) {
+          conv = true;
+          break;
+        }
+      }
+    }
+    if (conv || depth >= BS_MAXDEPTH) {
+      if (!conv) st |= ST_BS_DEPTH;
+      for (int i = 0; i < 6; ++i) yv[i] = cur[i];
+      t += H;
+    } else {
+      H *= 0.5;
+      ++depth;
+    }
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Massive subsystem (NM bodies of one system, state body-major xyz)
+// ---------------------------------------------------------------------------
+
+// Kick acceleration of every massive body from the others, pair changeover
+// radius max(r_c,j, r_c,k) (body_interaction_kick over massive partners,
+// src/sim.cuh:327-382, with self-exclusion by body index).
+template <int NM>
+__device__ __forceinline__ void accel_massive(const double *X, const double *__restrict__ prm,
+                                              double eps2, double *A) {
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      if (k == j) continue;
+      const double dx = X[3 * k + 0] - X[3 * j + 0];
+      const double dy = X[3 * k + 1] - X[3 * j + 1];
+      const double dz = X[3 * k + 2] - X[3 * j + 2];
+      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double w = kick_weight(r2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
+                                   prm[P_PRCWINV(NM) + j * NM + k]);
+      ax = fma(w, dx, ax);
+      ay = fma(w, dy, ay);
+      az = fma(w, dz, az);
+    }
+    A[3 * j + 0] = ax;
+    A[3 * j + 1] = ay;
+    A[3 * j + 2] = az;
+  }
+}
+
+// Q = (sum_j m_j v_j) / M  (main_body_kinetic's momentum sum, src/sim.cuh:384-406)
+template <int NM>
+__device__ __forceinline__ void momentum_q(const double *V, const double *__restrict__ prm,
+                                           double M, double *Q) {
+  double px = 0.0, py = 0.0, pz = 0.0;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    const double m = prm[P_M(NM) + j];
+    px = fma(m, V[3 * j + 0], px);
+    py = fma(m, V[3 * j + 1], py);
+    pz = fma(m, V[3 * j + 2], pz);
+  }
+  Q[0] = px / M;
+  Q[1] = py / M;
+  Q[2] = pz / M;
+}
+
+template <int NM>
+__device__ __forceinline__ bool encounter_massive(const double *X, const double *__restrict__ prm) {
+  bool f = false;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+#pragma unroll
+    for (int k = j + 1; k < NM; ++k) {
+      const double dx = X[3 * k + 0] - X[3 * j + 0];
+      const double dy = X[3 * k + 1] - X[3 * j + 1];
+      const double dz = X[3 * k + 2] - X[3 * j + 2];
+      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+      f = f || (r2 < prm[P_PRC2(NM) + j * NM + k]);
+    }
+  }
+  return f;
+}
+
+// Close-encounter RHS of the whole massive set (state [NM][6]).
+template <int NM>
+__device__ __forceinline__ void rhs_massive(const double *s, const double *__restrict__ prm,
+                                            double mu, double eps2, double *f) {
+  for (int j = 0; j < NM; ++j) {
+    const double x = s[6 * j + 0], y = s[6 * j + 1], z = s[6 * j + 2];
+    f[6 * j + 0] = s[6 * j + 3];
+    f[6 * j + 1] = s[6 * j + 4];
+    f[6 * j + 2] = s[6 * j + 5];
+    const double r2 = dot3(x, y, z, x, y, z);
+    const double r = sqrt(r2);
+    const double w0 = mu / (r2 * r);
+    double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
+    for (int k = 0; k < NM; ++k) {
+      if (k == j) continue;
+      const double dx = s[6 * k + 0] - x, dy = s[6 * k + 1] - y, dz = s[6 * k + 2] - z;
+      const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double w = close_weight(d2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
+                                    prm[P_PRCWINV(NM) + j * NM + k]);
+      ax = fma(w, dx, ax);
+      ay = fma(w, dy, ay);
+      az = fma(w, dz, az);
+    }
+    f[6 * j + 3] = ax;
+    f[6 * j + 4] = ay;
+    f[6 * j + 5] = az;
+  }
+}
+
+template <int NM>
+__device__ __noinline__ void gragg_massive(const double *y0, const double *f0, double H, int N,
+                                           const double *__restrict__ prm, double mu,
+                                           double eps2, double *out) {
+  constexpr int D = 6 * NM;
+  const double hs = H / (double)N;
+  const double h2 = 2.0 * hs;
+  double z0[D], z1[D], f[D];
+  for (int i = 0; i < D; ++i) {
+    z0[i] = y0[i];
+    z1[i] = fma(hs, f0[i], y0[i]);
+  }
+  for (int m = 1; m < N; ++m) {
+    rhs_massive<NM>(z1, prm, mu, eps2, f);
+    for (int i = 0; i < D; ++i) {
+      const double z2 = fma(h2, f[i], z0[i]);
+      z0[i] = z1[i];
+      z1[i] = z2;
+    }
+  }
+  rhs_massive<NM>(z1, prm, mu, eps2, f);
+  for (int i = 0; i < D; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
+}
+
+// BS drift of the whole massive set when any massive pair is inside its
+// changeover radius.  Error scale per body: |x_j|, |v_j|.
+template <int NM>
+__device__ __noinline__ uint32_t bs_drift_massive(double *s, double h,
+                                                  const double *__restrict__ prm, double mu,
+                                                  double eps2, double tol) {
+  constexpr int D = 6 * NM;
+  uint32_t st = 0;
+  double t = 0.0, H = h;
+  int depth = 0;
+  while (t < h) {
+    if (H > h - t) H = h - t;
+    double f0[D], sc[2 * NM];
+    rhs_massive<NM>(s, prm, mu, eps2, f0);
+    for (int j = 0; j < NM; ++j) {
+      sc[2 * j] = sqrt(dot3(s[6 * j], s[6 * j + 1], s[6 * j + 2], s[6 * j], s[6 * j + 1], s[6 * j + 2]));
+      sc[2 * j + 1] =
+          sqrt(dot3(s[6 * j + 3], s[6 * j + 4], s[6 * j + 5], s[6 * j + 3], s[6 * j + 4], s[6 * j + 5]));
+    }
+    double R[KMAX][D];
+    double cur[D];
+    bool conv = false;
+    for (int k = 0; k < KMAX; ++k) {
+      gragg_massive<NM>(s, f0, H, 2 << k, prm, mu, eps2, cur);
+      for (int j = 1; j <= k; ++j) {
+        const double fac = bs_fac(j);
+        for (int i = 0; i < D; ++i) {
+          const double prev = R[j - 1][i];
+          R[j - 1][i] = cur[i];
+          cur[i] = cur[i] + (cur[i] - prev) / fac;
+        }
+      }
+      for (int i = 0; i < D; ++i) R[k][i] = cur[i];
+      if (k >= 1) {
+        double err = 0.0;
+        for (int i = 0; i < D; ++i) {
+          const int b = i / 6, c = i % 6;
+          const double e = fabs(cur[i] - R[k - 1][i]) / sc[2 * b + (c < 3 ? 0 : 1)];
+          err = e > err ? e : err;
+        }
+        if (err <= tol) {
+          conv = true;
+          break;
+        }
+      }
+    }
+    if (conv || depth >= BS_MAXDEPTH) {
+      if (!conv) st |= ST_BS_DEPTH;
+      for (int i = 0; i < D; ++i) s[i] = cur[i];
+      t += H;
+    } else {
+      H *= 0.5;
+      ++depth;
+    }
+  }
+  return st;
+}
+
+}  // namespace mint

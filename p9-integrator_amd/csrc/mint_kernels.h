@@ -1,0 +1,511 @@
+// mint_kernels.h — gfx950 kernels of the Mercurius step.
+//
+//  ephemeris_kernel   massive subsystem of each system for C steps; one lane
+//                     per system; writes the per-step ephemeris records that
+//                     every TP phase reads (replaces the cluster-wide reads of
+//                     block 0's shared memory, src/sim.cuh:15-26,682-741)
+//  tp_step_kernel     one step for one TP per lane (plain and flat forms)
+//  tp_main_kernel     C steps per launch with TP state in registers; a TP that
+//                     fails the encounter check is appended to the BS queue and
+//                     its lane goes idle for the rest of the launch
+//  tp_encounter_kernel  consumes the BS queue: each lane takes one deferred TP
+//                     and carries it to the end of the launch window
+#pragma once
+#include "mint_device.h"
+#include "mint_nn.h"
+
+namespace mint {
+
+struct Tile {
+  int sys;
+  int count;
+  long long start;
+};
+
+// One deferred TP (64 B).  phase 0: state at the start of step k0 (before the
+// opening kick); phase 1: state after jump 1 of step k0, encounter flagged.
+struct QEntry {
+  int tp;
+  int sys;
+  int k0;
+  int phase;
+  double s[6];
+};
+
+struct TPArrays {
+  double *x, *y, *z, *vx, *vy, *vz;
+};
+
+// stats slots (unsigned long long)
+enum {
+  S_TPSTEPS = 0,
+  S_KEPLER = 1,
+  S_NEWTON = 2,
+  S_BSSTEPS = 3,
+  S_BSRHS = 4,
+  S_DEFER = 5,
+  S_PREBUCKET = 6,
+  S_IDLE = 7,
+  S_COUNT = 8
+};
+
+__device__ __forceinline__ void q_put(QEntry *e, int tp, int sys, int k0, int phase, double x,
+                                      double y, double z, double vx, double vy, double vz) {
+  e->tp = tp;
+  e->sys = sys;
+  e->k0 = k0;
+  e->phase = phase;
+  e->s[0] = x;
+  e->s[1] = y;
+  e->s[2] = z;
+  e->s[3] = vx;
+  e->s[4] = vy;
+  e->s[5] = vz;
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v != 0u) atomicAdd(dst, (unsigned long long)v);
+}
+
+__device__ __forceinline__ bool finite6(double a, double b, double c, double d, double e,
+                                        double f) {
+  return isfinite(a) && isfinite(b) && isfinite(c) && isfinite(d) && isfinite(e) && isfinite(f);
+}
+
+// Kepler-MLP input features of one drift (FP32).  tau = dt sqrt(mu/r0^3).
+__device__ __forceinline__ void kepler_features(const KepStart &k, double mu, double dt,
+                                                float *f, double &tau) {
+  tau = dt * sqrt(mu / (k.r0 * (k.r0 * k.r0)));
+  f[0] = (log2f((float)tau) + 6.0f) * 0.25f;
+  f[1] = (float)(k.zeta0 / mu);
+  f[2] = (float)(k.eta0 / sqrt(mu * k.r0));
+  float ra = (float)(k.r0 * k.beta / mu);
+  f[3] = fminf(fmaxf(ra, -2.0f), 2.0f);
+}
+
+// Starting guess of the TP Kepler solve.  GUESS: 0 crude, 1 series, 2 learned
+// (series guess times (1 + c tau^3), c from the MLP, where tau > KM_TAU_GATE).
+template <int GUESS>
+__device__ __forceinline__ double tp_guess(const KepStart &k, double mu, double dt,
+                                           const float *__restrict__ nnw) {
+  if (GUESS == 0) return guess_crude(k, dt);
+  double s = guess_series(k, dt);
+  if (GUESS == 2) {
+    const double tau0 = dt * sqrt(mu / (k.r0 * (k.r0 * k.r0)));
+    if (tau0 > KM_TAU_GATE) {
+      float f[4];
+      double tau;
+      kepler_features(k, mu, dt, f, tau);
+      const double c = (double)mlp_kepler_fwd(nnw, f);
+      s = s * fma(c, tau * (tau * tau), 1.0);
+    }
+  }
+  return s;
+}
+
+// TP drift over the step: Kepler unless flagged.  Returns Newton iterations.
+template <int GUESS>
+__device__ __forceinline__ int tp_kepler(double &x, double &y, double &z, double &vx, double &vy,
+                                         double &vz, const StepConst &sc,
+                                         const float *__restrict__ nnw, uint32_t &st) {
+  const KepStart k = kep_start(x, y, z, vx, vy, vz, sc.mu);
+  const double s0 = tp_guess<GUESS>(k, sc.mu, sc.h, nnw);
+  bool ok;
+  const int it = kepler_drift_from(k, s0, x, y, z, vx, vy, vz, sc.mu, sc.h, ok);
+  if (!ok) st |= ST_NEWTON;
+  return it;
+}
+
+// ---------------------------------------------------------------------------
+template <int NM>
+__global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, int C,
+                                                       const double *__restrict__ prm_all,
+                                                       double *__restrict__ mx,
+                                                       double *__restrict__ mv,
+                                                       double *__restrict__ xs0,
+                                                       double *__restrict__ rec,
+                                                       unsigned int *__restrict__ status,
+                                                       StepConst sc, double M) {
+  constexpr int REC = rec_size(NM);
+  const int s = s_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= s_end) return;
+  const double *prm = prm_all + (size_t)s * prm_size(NM);
+  double X[3 * NM], V[3 * NM], A[3 * NM], Q[3];
+#pragma unroll
+  for (int i = 0; i < 3 * NM; ++i) {
+    X[i] = mx[(size_t)s * 3 * NM + i];
+    V[i] = mv[(size_t)s * 3 * NM + i];
+    xs0[(size_t)s * 3 * NM + i] = X[i];
+  }
+  accel_massive<NM>(X, prm, sc.eps2, A);
+  uint32_t st = 0;
+  for (int k = 0; k < C; ++k) {
+    double *r = rec + ((size_t)s * C + k) * REC;
+#pragma unroll
+    for (int i = 0; i < 3 * NM; ++i) V[i] = fma(sc.hh, A[i], V[i]);
+    momentum_q<NM>(V, prm, M, Q);
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[3 * j + c] = fma(sc.hh, Q[c], X[3 * j + c]);
+    r[R_Q1(NM) + 0] = Q[0];
+    r[R_Q1(NM) + 1] = Q[1];
+    r[R_Q1(NM) + 2] = Q[2];
+#pragma unroll
+    for (int i = 0; i < 3 * NM; ++i) {
+      r[R_Y(NM) + i] = X[i];
+      r[R_W(NM) + i] = V[i];
+    }
+    if (encounter_massive<NM>(X, prm)) {
+      st |= ST_MASSIVE_ENC;
+      double sv[6 * NM];
+      for (int j = 0; j < NM; ++j)
+        for (int c = 0; c < 3; ++c) {
+          sv[6 * j + c] = X[3 * j + c];
+          sv[6 * j + 3 + c] = V[3 * j + c];
+        }
+      st |= bs_drift_massive<NM>(sv, sc.h, prm, sc.mu, sc.eps2, sc.tol);
+      for (int j = 0; j < NM; ++j)
+        for (int c = 0; c < 3; ++c) {
+          X[3 * j + c] = sv[6 * j + c];
+          V[3 * j + c] = sv[6 * j + 3 + c];
+        }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NM; ++j) {
+        bool ok;
+        kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
+                            V[3 * j + 2], sc.mu, sc.h, ok);
+        if (!ok) st |= ST_NEWTON;
+      }
+    }
+    momentum_q<NM>(V, prm, M, Q);
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[3 * j + c] = fma(sc.hh, Q[c], X[3 * j + c]);
+    r[R_Q2(NM) + 0] = Q[0];
+    r[R_Q2(NM) + 1] = Q[1];
+    r[R_Q2(NM) + 2] = Q[2];
+    accel_massive<NM>(X, prm, sc.eps2, A);
+#pragma unroll
+    for (int i = 0; i < 3 * NM; ++i) {
+      V[i] = fma(sc.hh, A[i], V[i]);
+      r[R_XE(NM) + i] = X[i];
+    }
+  }
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < 3 * NM; ++i) {
+    mx[(size_t)s * 3 * NM + i] = X[i];
+    mv[(size_t)s * 3 * NM + i] = V[i];
+    fin = fin && isfinite(X[i]) && isfinite(V[i]);
+  }
+  if (!fin) st |= ST_NAN;
+  if (st) atomicOr(&status[s], st);
+}
+
+// ---------------------------------------------------------------------------
+// One full step of one TP (plain / flat forms).  The ephemeris buffer holds
+// one record per system (C = 1).
+template <int NM, int GUESS>
+__global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1,
+                                                      const int *__restrict__ tp_sys,
+                                                      TPArrays tp,
+                                                      const double *__restrict__ prm_all,
+                                                      const double *__restrict__ xs0,
+                                                      const double *__restrict__ rec,
+                                                      StepConst sc,
+                                                      const float *__restrict__ nnw,
+                                                      unsigned int *__restrict__ status,
+                                                      unsigned long long *__restrict__ stats) {
+  constexpr int REC = rec_size(NM);
+  const long long i = i0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int n_kep = 0, n_newton = 0, n_bs = 0, n_rhs = 0, n_step = 0;
+  if (i < i1) {
+    const int s = tp_sys[i];
+    const double *prm = prm_all + (size_t)s * prm_size(NM);
+    const double *r = rec + (size_t)s * REC;
+    double x = tp.x[i], y = tp.y[i], z = tp.z[i];
+    double vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
+    uint32_t st = 0;
+    double ax, ay, az;
+    accel_tp<NM>(x, y, z, xs0 + (size_t)s * 3 * NM, prm, sc.eps2, ax, ay, az);
+    vx = fma(sc.hh, ax, vx);
+    vy = fma(sc.hh, ay, vy);
+    vz = fma(sc.hh, az, vz);
+    x = fma(sc.hh, r[R_Q1(NM) + 0], x);
+    y = fma(sc.hh, r[R_Q1(NM) + 1], y);
+    z = fma(sc.hh, r[R_Q1(NM) + 2], z);
+    if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
+      double yv[6] = {x, y, z, vx, vy, vz};
+      int nr = 0;
+      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+      x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
+      n_bs = 1;
+      n_rhs = (unsigned)nr;
+    } else {
+      n_newton = (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
+      n_kep = 1;
+    }
+    x = fma(sc.hh, r[R_Q2(NM) + 0], x);
+    y = fma(sc.hh, r[R_Q2(NM) + 1], y);
+    z = fma(sc.hh, r[R_Q2(NM) + 2], z);
+    accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az);
+    vx = fma(sc.hh, ax, vx);
+    vy = fma(sc.hh, ay, vy);
+    vz = fma(sc.hh, az, vz);
+    tp.x[i] = x; tp.y[i] = y; tp.z[i] = z;
+    tp.vx[i] = vx; tp.vy[i] = vy; tp.vz[i] = vz;
+    if (!finite6(x, y, z, vx, vy, vz)) st |= ST_NAN;
+    if (st) atomicOr(&status[s], st);
+    n_step = 1;
+  }
+  if (stats) {
+    wave_add(stats + S_TPSTEPS, n_step);
+    wave_add(stats + S_KEPLER, n_kep);
+    wave_add(stats + S_NEWTON, n_newton);
+    wave_add(stats + S_BSSTEPS, n_bs);
+    wave_add(stats + S_BSRHS, n_rhs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scorer features of one (TP, massive body) pair at the start of a launch
+// window of duration T: distance, approach rate and closest approach in units
+// of r_c, and Hill-zone size ratios.
+__device__ __forceinline__ void scorer_features(double x, double y, double z, double vx, double vy,
+                                                double vz, double X, double Y, double Z, double VX,
+                                                double VY, double VZ, double rc, double T,
+                                                float *f) {
+  const double dx = X - x, dy = Y - y, dz = Z - z;
+  const double ux = VX - vx, uy = VY - vy, uz = VZ - vz;
+  const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+  const double u2 = dot3(ux, uy, uz, ux, uy, uz);
+  const double du = dot3(dx, dy, dz, ux, uy, uz);
+  double tmin = u2 > 0.0 ? -du / u2 : 0.0;
+  tmin = fmin(fmax(tmin, 0.0), T);
+  const double mx = fma(ux, tmin, dx), my = fma(uy, tmin, dy), mz = fma(uz, tmin, dz);
+  const double rmin = sqrt(dot3(mx, my, mz, mx, my, mz));
+  const double R = sqrt(dot3(X, Y, Z, X, Y, Z));
+  const double rt = sqrt(dot3(x, y, z, x, y, z));
+  f[0] = (float)fmin(r / rc, 64.0) * 0.125f;
+  f[1] = (float)fmax(fmin(du / (r * rc) * T, 16.0), -16.0) * 0.25f;
+  f[2] = (float)fmin(sqrt(u2) * T / rc, 16.0) * 0.25f;
+  f[3] = (float)fmin(rmin / rc, 64.0) * 0.125f;
+  f[4] = (float)(rc / R) * 4.0f;
+  f[5] = (float)fmin(rt / R, 8.0) * 0.25f;
+}
+
+template <int NM, int P, int GUESS>
+__global__ void __launch_bounds__(1024) tp_main_kernel(
+    const Tile *__restrict__ tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
+    const double *__restrict__ prm_all, const double *__restrict__ xs0,
+    const double *__restrict__ rec, int C, StepConst sc, const float *__restrict__ nnw,
+    const float *__restrict__ scw, float score_thresh, QEntry *__restrict__ q,
+    unsigned int *__restrict__ qcount, unsigned int *__restrict__ status,
+    unsigned long long *__restrict__ stats) {
+  constexpr int REC = rec_size(NM);
+  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so the tiles of one system share an L2.
+  const int b = blockIdx.x;
+  const int t_idx = (b & 7) * tiles_per_xcd + (b >> 3);
+  if (t_idx >= ntiles) return;
+  const Tile tile = tiles[t_idx];
+  const int s = tile.sys;
+  const double *prm = prm_all + (size_t)s * prm_size(NM);
+  const double *xsys = xs0 + (size_t)s * 3 * NM;
+  const double *rsys = rec + (size_t)s * C * REC;
+
+  double X[P][3], V[P][3], A[P][3];
+  bool act[P];
+  long long idx[P];
+  unsigned int n_kep = 0, n_newton = 0, n_defer = 0, n_pre = 0, n_idle = 0, n_step = 0;
+  uint32_t st = 0;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int loc = p * (int)blockDim.x + (int)threadIdx.x;
+    act[p] = loc < tile.count;
+    idx[p] = tile.start + loc;
+    if (act[p]) {
+      X[p][0] = tp.x[idx[p]]; X[p][1] = tp.y[idx[p]]; X[p][2] = tp.z[idx[p]];
+      V[p][0] = tp.vx[idx[p]]; V[p][1] = tp.vy[idx[p]]; V[p][2] = tp.vz[idx[p]];
+    }
+  }
+  if (scw != nullptr) {
+    // Pre-bucket TPs the scorer expects to meet a changeover zone in this
+    // window; the encounter kernel then runs them from step 0.  The exact
+    // distance check still decides every step, so results do not change.
+    const double T = sc.h * (double)C;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (!act[p]) continue;
+      float best = -1e30f;
+      for (int j = 0; j < NM; ++j) {
+        float f[SC_IN];
+        const double rc = sqrt(prm[P_RC2(NM) + j]);
+        scorer_features(X[p][0], X[p][1], X[p][2], V[p][0], V[p][1], V[p][2], xsys[3 * j],
+                        xsys[3 * j + 1], xsys[3 * j + 2], rsys[R_W(NM) + 3 * j],
+                        rsys[R_W(NM) + 3 * j + 1], rsys[R_W(NM) + 3 * j + 2], rc, T, f);
+        best = fmaxf(best, scorer_fwd(scw, f));
+      }
+      if (best > score_thresh) {
+        const unsigned int slot = atomicAdd(qcount, 1u);
+        q_put(q + slot, (int)idx[p], s, 0, 0, X[p][0], X[p][1], X[p][2], V[p][0], V[p][1],
+              V[p][2]);
+        act[p] = false;
+        ++n_pre;
+        n_idle += (unsigned)C;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+    if (act[p]) accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2]);
+
+  for (int k = 0; k < C; ++k) {
+    const double *r = rsys + (size_t)k * REC;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (!act[p]) continue;
+      double x = X[p][0], y = X[p][1], z = X[p][2];
+      double vx = fma(sc.hh, A[p][0], V[p][0]);
+      double vy = fma(sc.hh, A[p][1], V[p][1]);
+      double vz = fma(sc.hh, A[p][2], V[p][2]);
+      x = fma(sc.hh, r[R_Q1(NM) + 0], x);
+      y = fma(sc.hh, r[R_Q1(NM) + 1], y);
+      z = fma(sc.hh, r[R_Q1(NM) + 2], z);
+      if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
+        const unsigned int slot = atomicAdd(qcount, 1u);
+        q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
+        act[p] = false;
+        ++n_defer;
+        n_idle += (unsigned)(C - k);
+        continue;
+      }
+      n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
+      ++n_kep;
+      x = fma(sc.hh, r[R_Q2(NM) + 0], x);
+      y = fma(sc.hh, r[R_Q2(NM) + 1], y);
+      z = fma(sc.hh, r[R_Q2(NM) + 2], z);
+      double ax, ay, az;
+      accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az);
+      A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
+      X[p][0] = x; X[p][1] = y; X[p][2] = z;
+      V[p][0] = fma(sc.hh, ax, vx);
+      V[p][1] = fma(sc.hh, ay, vy);
+      V[p][2] = fma(sc.hh, az, vz);
+      ++n_step;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    if (!act[p]) continue;
+    tp.x[idx[p]] = X[p][0]; tp.y[idx[p]] = X[p][1]; tp.z[idx[p]] = X[p][2];
+    tp.vx[idx[p]] = V[p][0]; tp.vy[idx[p]] = V[p][1]; tp.vz[idx[p]] = V[p][2];
+    if (!finite6(X[p][0], X[p][1], X[p][2], V[p][0], V[p][1], V[p][2])) st |= ST_NAN;
+  }
+  if (st) atomicOr(&status[s], st);
+  if (stats) {
+    wave_add(stats + S_TPSTEPS, n_step);
+    wave_add(stats + S_KEPLER, n_kep);
+    wave_add(stats + S_NEWTON, n_newton);
+    wave_add(stats + S_DEFER, n_defer);
+    wave_add(stats + S_PREBUCKET, n_pre);
+    wave_add(stats + S_IDLE, n_idle);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BS queue consumer.  Lanes take entries with an atomic head counter so long
+// and short jobs balance across the grid; each entry is carried from its step
+// k0 to the end of the window, with the exact check deciding Kepler or BS at
+// every step.
+template <int NM, int GUESS>
+__global__ void __launch_bounds__(256) tp_encounter_kernel(
+    const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
+    unsigned int *__restrict__ qhead, TPArrays tp, const double *__restrict__ prm_all,
+    const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
+    const float *__restrict__ nnw, unsigned int *__restrict__ status,
+    unsigned long long *__restrict__ stats) {
+  constexpr int REC = rec_size(NM);
+  const unsigned int n = *qcount;
+  unsigned int n_kep = 0, n_newton = 0, n_bs = 0, n_rhs = 0, n_step = 0;
+  for (;;) {
+    const unsigned int e = atomicAdd(qhead, 1u);
+    if (e >= n) break;
+    const QEntry en = q[e];
+    const int s = en.sys;
+    const double *prm = prm_all + (size_t)s * prm_size(NM);
+    const double *rsys = rec + (size_t)s * C * REC;
+    double x = en.s[0], y = en.s[1], z = en.s[2], vx = en.s[3], vy = en.s[4], vz = en.s[5];
+    double ax, ay, az;
+    uint32_t st = 0;
+    int k = en.k0;
+    bool flag = true;
+    if (en.phase == 0) {
+      const double *xs = (k == 0) ? xs0 + (size_t)s * 3 * NM : rsys + (size_t)(k - 1) * REC + R_XE(NM);
+      accel_tp<NM>(x, y, z, xs, prm, sc.eps2, ax, ay, az);
+      const double *r = rsys + (size_t)k * REC;
+      vx = fma(sc.hh, ax, vx);
+      vy = fma(sc.hh, ay, vy);
+      vz = fma(sc.hh, az, vz);
+      x = fma(sc.hh, r[R_Q1(NM) + 0], x);
+      y = fma(sc.hh, r[R_Q1(NM) + 1], y);
+      z = fma(sc.hh, r[R_Q1(NM) + 2], z);
+      flag = encounter_tp<NM>(x, y, z, r + R_Y(NM), prm);
+    }
+    for (;;) {
+      const double *r = rsys + (size_t)k * REC;
+      if (flag) {
+        double yv[6] = {x, y, z, vx, vy, vz};
+        int nr = 0;
+        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+        x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
+        ++n_bs;
+        n_rhs += (unsigned)nr;
+      } else {
+        n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
+        ++n_kep;
+      }
+      x = fma(sc.hh, r[R_Q2(NM) + 0], x);
+      y = fma(sc.hh, r[R_Q2(NM) + 1], y);
+      z = fma(sc.hh, r[R_Q2(NM) + 2], z);
+      accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az);
+      vx = fma(sc.hh, ax, vx);
+      vy = fma(sc.hh, ay, vy);
+      vz = fma(sc.hh, az, vz);
+      ++n_step;
+      if (++k >= C) break;
+      const double *rn = rsys + (size_t)k * REC;
+      vx = fma(sc.hh, ax, vx);
+      vy = fma(sc.hh, ay, vy);
+      vz = fma(sc.hh, az, vz);
+      x = fma(sc.hh, rn[R_Q1(NM) + 0], x);
+      y = fma(sc.hh, rn[R_Q1(NM) + 1], y);
+      z = fma(sc.hh, rn[R_Q1(NM) + 2], z);
+      flag = encounter_tp<NM>(x, y, z, rn + R_Y(NM), prm);
+    }
+    const long long i = en.tp;
+    tp.x[i] = x; tp.y[i] = y; tp.z[i] = z;
+    tp.vx[i] = vx; tp.vy[i] = vy; tp.vz[i] = vz;
+    if (!finite6(x, y, z, vx, vy, vz)) st |= ST_NAN;
+    if (st) atomicOr(&status[s], st);
+  }
+  if (stats) {
+    wave_add(stats + S_TPSTEPS, n_step);
+    wave_add(stats + S_KEPLER, n_kep);
+    wave_add(stats + S_NEWTON, n_newton);
+    wave_add(stats + S_BSSTEPS, n_bs);
+    wave_add(stats + S_BSRHS, n_rhs);
+  }
+}
+
+static __global__ void zero_u32_kernel(unsigned int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+}  // namespace mint

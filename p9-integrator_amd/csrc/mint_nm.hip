@@ -1,0 +1,113 @@
+// mint_nm.hip — kernel instantiations for one massive-body count, MINT_NM.
+#include "mint_launch.h"
+
+#ifndef MINT_NM
+#error "build with -DMINT_NM=<1..8>"
+#endif
+
+namespace mint {
+namespace {
+
+constexpr int NMV = MINT_NM;
+
+hipError_t ephemeris_l(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
+                       double *mx, double *mv, double *xs0, double *rec, unsigned int *status,
+                       StepConst sc, double M) {
+  const int n = s_end - s_begin;
+  if (n <= 0) return hipSuccess;
+  ephemeris_kernel<NMV><<<(n + 63) / 64, 64, 0, st>>>(s_begin, s_end, C, prm, mx, mv, xs0, rec,
+                                                      status, sc, M);
+  return hipGetLastError();
+}
+
+hipError_t tp_step_l(hipStream_t st, int guess, int block, long long i0, long long i1,
+                     const int *tp_sys, TPArrays tp, const double *prm, const double *xs0,
+                     const double *rec, StepConst sc, const float *nnw, unsigned int *status,
+                     unsigned long long *stats) {
+  const long long n = i1 - i0;
+  if (n <= 0) return hipSuccess;
+  if (block <= 0 || block > 256 || (block & 63)) return hipErrorInvalidValue;
+  const unsigned int grid = (unsigned int)((n + block - 1) / block);
+  switch (guess) {
+    case 0:
+      tp_step_kernel<NMV, 0><<<grid, block, 0, st>>>(i0, i1, tp_sys, tp, prm, xs0, rec, sc, nnw,
+                                                     status, stats);
+      break;
+    case 1:
+      tp_step_kernel<NMV, 1><<<grid, block, 0, st>>>(i0, i1, tp_sys, tp, prm, xs0, rec, sc, nnw,
+                                                     status, stats);
+      break;
+    case 2:
+      tp_step_kernel<NMV, 2><<<grid, block, 0, st>>>(i0, i1, tp_sys, tp, prm, xs0, rec, sc, nnw,
+                                                     status, stats);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P, int G>
+hipError_t main_pg(hipStream_t st, int block, int nblocks, const Tile *tiles, int ntiles,
+                   int tiles_per_xcd, TPArrays tp, const double *prm, const double *xs0,
+                   const double *rec, int C, StepConst sc, const float *nnw, const float *scw,
+                   float thresh, QEntry *q, unsigned int *qcount, unsigned int *status,
+                   unsigned long long *stats) {
+  tp_main_kernel<NMV, P, G><<<nblocks, block, 0, st>>>(tiles, ntiles, tiles_per_xcd, tp, prm, xs0,
+                                                       rec, C, sc, nnw, scw, thresh, q, qcount,
+                                                       status, stats);
+  return hipGetLastError();
+}
+
+hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, const Tile *tiles,
+                     int ntiles, int tiles_per_xcd, TPArrays tp, const double *prm,
+                     const double *xs0, const double *rec, int C, StepConst sc, const float *nnw,
+                     const float *scw, float thresh, QEntry *q, unsigned int *qcount,
+                     unsigned int *status, unsigned long long *stats) {
+  if (block <= 0 || block > 1024 || (block & 63)) return hipErrorInvalidValue;
+#define MINT_MAIN(PP, GG)                                                                       \
+  if (P == PP && guess == GG)                                                                   \
+    return main_pg<PP, GG>(st, block, nblocks, tiles, ntiles, tiles_per_xcd, tp, prm, xs0, rec, \
+                           C, sc, nnw, scw, thresh, q, qcount, status, stats);
+  MINT_MAIN(1, 0) MINT_MAIN(1, 1) MINT_MAIN(1, 2)
+  MINT_MAIN(2, 0) MINT_MAIN(2, 1) MINT_MAIN(2, 2)
+  MINT_MAIN(4, 0) MINT_MAIN(4, 1) MINT_MAIN(4, 2)
+#undef MINT_MAIN
+  return hipErrorInvalidValue;
+}
+
+hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
+                          const unsigned int *qcount, unsigned int *qhead, TPArrays tp,
+                          const double *prm, const double *xs0, const double *rec, int C,
+                          StepConst sc, const float *nnw, unsigned int *status,
+                          unsigned long long *stats) {
+  if (block <= 0 || block > 256 || (block & 63)) return hipErrorInvalidValue;
+  switch (guess) {
+    case 0:
+      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
+                                                          sc, nnw, status, stats);
+      break;
+    case 1:
+      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
+                                                          sc, nnw, status, stats);
+      break;
+    case 2:
+      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
+                                                          sc, nnw, status, stats);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define MINT_CAT2(a, b) a##b
+#define MINT_CAT(a, b) MINT_CAT2(a, b)
+const Launchers *MINT_CAT(launchers_nm, MINT_NM)() {
+  static const Launchers L = {ephemeris_l, tp_step_l, tp_main_l, tp_encounter_l};
+  return &L;
+}
+
+}  // namespace mint

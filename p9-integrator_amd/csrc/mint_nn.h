@@ -1,0 +1,67 @@
+// mint_nn.h — FP32 networks of the learned parts (device side).
+//
+// Kepler warm-start MLP: 4 -> 32 -> 32 -> 1, tanh hidden layers, linear out.
+// Encounter scorer:      6 -> 16 -> 1, tanh hidden layer, logit out.
+// Weights are one flat float array per network, uniform across a launch, so
+// the compiler reads them with scalar loads.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mint {
+
+// Flat layouts.
+constexpr int KM_IN = 4, KM_H = 32;
+constexpr int KM_W1 = 0;                       // [KM_H][KM_IN]
+constexpr int KM_B1 = KM_W1 + KM_H * KM_IN;    // [KM_H]
+constexpr int KM_W2 = KM_B1 + KM_H;            // [KM_H][KM_H]
+constexpr int KM_B2 = KM_W2 + KM_H * KM_H;     // [KM_H]
+constexpr int KM_W3 = KM_B2 + KM_H;            // [KM_H]
+constexpr int KM_B3 = KM_W3 + KM_H;            // [1]
+constexpr int KM_NP = KM_B3 + 1;
+
+constexpr int SC_IN = 6, SC_H = 16;
+constexpr int SC_W1 = 0;                       // [SC_H][SC_IN]
+constexpr int SC_B1 = SC_W1 + SC_H * SC_IN;    // [SC_H]
+constexpr int SC_W2 = SC_B1 + SC_H;            // [SC_H]
+constexpr int SC_B2 = SC_W2 + SC_H;            // [1]
+constexpr int SC_NP = SC_B2 + 1;
+
+// Learned guess is used only where the series guess is expected to need a
+// second Newton iteration: local dimensionless step tau above this gate.
+constexpr double KM_TAU_GATE = 0.02;
+
+__device__ __forceinline__ float nn_tanh(float x) { return tanhf(x); }
+
+__device__ __forceinline__ float mlp_kepler_fwd(const float *__restrict__ w, const float *in) {
+  float h1[KM_H];
+#pragma unroll
+  for (int i = 0; i < KM_H; ++i) {
+    float a = w[KM_B1 + i];
+#pragma unroll
+    for (int k = 0; k < KM_IN; ++k) a = fmaf(w[KM_W1 + i * KM_IN + k], in[k], a);
+    h1[i] = nn_tanh(a);
+  }
+  float out = w[KM_B3];
+#pragma unroll 4
+  for (int i = 0; i < KM_H; ++i) {
+    float a = w[KM_B2 + i];
+#pragma unroll
+    for (int k = 0; k < KM_H; ++k) a = fmaf(w[KM_W2 + i * KM_H + k], h1[k], a);
+    out = fmaf(w[KM_W3 + i], nn_tanh(a), out);
+  }
+  return out;
+}
+
+__device__ __forceinline__ float scorer_fwd(const float *__restrict__ w, const float *in) {
+  float out = w[SC_B2];
+#pragma unroll
+  for (int i = 0; i < SC_H; ++i) {
+    float a = w[SC_B1 + i];
+#pragma unroll
+    for (int k = 0; k < SC_IN; ++k) a = fmaf(w[SC_W1 + i * SC_IN + k], in[k], a);
+    out = fmaf(w[SC_W2 + i], nn_tanh(a), out);
+  }
+  return out;
+}
+
+}  // namespace mint
