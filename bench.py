@@ -1,0 +1,202 @@
+"""Benchmark: test-particle steps per second on the 4096-system Planet Nine
+sweep (dt = 0.5 yr), one MI355X, through libmint's C ABI.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--form learned|batched|flat|plain]
+
+A "step" is one Mercurius step of every system of the sweep (about 5.6 M test
+particles, five massive bodies each).  The state is uploaded to HBM before the
+timed region; the timed region is exactly K steps, bracketed by device
+synchronisation (the library's stream event) on both sides.  One JSON line is
+printed with the roofline of the dominant kernel (tp_main_kernel, FP64 vector
+ALU bound) and the CPU oracle baseline timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "p9-integrator_amd"))
+
+import mint  # noqa: E402
+import mint_workloads as W  # noqa: E402
+
+METRIC = "test-particle steps/s, 4096-system P9 sweep, learned+batched vs plain, dE<=1e-9"
+FP64_PEAK = 78.6e12   # MI355X dense FP64 vector, FLOP/s
+HBM_PEAK = 8.0e12     # B/s
+N_M = 5
+
+
+def flops_per_tp_step(nbar: float, nm: int = N_M) -> float:
+    """Algorithmic FLOP per TP-step: two changeover half-kicks (28 per pair
+    each), two velocity updates (12), two jumps (12), the exact encounter check
+    (9 per body), and the universal-variable drift (60 + 40 per Newton
+    iteration); add/mul/div/sqrt = 1, fma = 2."""
+    return 2 * nm * 28 + 12 + 12 + 9 * nm + 60 + 40 * nbar
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_ctx(form, guess, scorer, stats=0, **kw):
+    return mint.Mint(dt=0.5, form=form, kepler_guess=guess, use_scorer=scorer, collect_stats=stats, **kw)
+
+
+def timed_run(g, st, steps, warmup):
+    """Warm up on the uploaded state, re-upload (same buffers, graphs stay
+    valid) and time exactly `steps` steps."""
+    g.upload(st)
+    if warmup > 0:
+        g.step(warmup)
+    g.upload(st)
+    E0 = g.energy()
+    g.reset_counters()
+    t0 = time.perf_counter()
+    g.step(steps)            # returns after the stream's closing event has completed
+    t1 = time.perf_counter()
+    tm = g.timing()
+    E1 = g.energy()
+    return (t1 - t0), tm["ms_total"] / 1e3, E0, E1
+
+
+def cpu_baseline(st_full, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # test infrastructure: the CPU baseline leg only
+
+    probe = st_full.subset([0])
+    t0 = time.perf_counter()
+    O.run(probe, 20, 0.5, guess=1)
+    rate = probe.n_tp * 20 / max(time.perf_counter() - t0, 1e-6)
+    steps = 100
+    nsys = int(max(1, min(st_full.n_systems, budget_s * rate / (steps * np.mean(np.diff(st_full.tp_offset))))))
+    sample = st_full.subset(np.arange(nsys))
+    t0 = time.perf_counter()
+    _, _, stats = O.run(sample, steps, 0.5, guess=1, threads=1)
+    dt = time.perf_counter() - t0
+    return dict(value=stats["tp_steps"] / dt, unit="TP-steps/s", cores=1, kind="port",
+                sample=f"systems 0-{nsys - 1} of the sweep ({sample.n_tp} TPs), {steps} steps, oracle single thread, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--form", default="batched", choices=["learned", "batched", "flat", "plain"])
+    ap.add_argument("--systems", type=int, default=4096)
+    ap.add_argument("--no-baseline", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the plain / flat comparison runs")
+    ap.add_argument("--plain-systems", type=int, default=64, help="systems in the plain-form sample")
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "p9-integrator_amd", "autotune.cache"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (systems would shard "
+                                   "across GPUs with no exchange, see the sharding paragraph of DESIGN.md)"}))
+        sys.exit(2)
+
+    log(f"bench: generating the {args.systems}-system sweep")
+    st = W.p9_sweep(S=4096, systems=np.arange(args.systems))
+    ntp = st.n_tp
+    log(f"bench: {ntp} TPs")
+
+    form = {"learned": mint.FORM_BATCHED, "batched": mint.FORM_BATCHED, "flat": mint.FORM_FLAT,
+            "plain": mint.FORM_PLAIN}[args.form]
+    guess = mint.GUESS_LEARNED if args.form == "learned" else mint.GUESS_SERIES
+    scorer = 1 if args.form == "learned" else 0
+
+    g = make_ctx(form, guess, scorer)
+    shape = None
+    if form == mint.FORM_BATCHED:
+        g.upload(st)
+        shape = g.autotune(args.tune_cache)
+        log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]}")
+    wall, ev_s, E0, E1 = timed_run(g, st, args.steps, args.warmup)
+    status = g.status()
+    value = ntp * args.steps / ev_s
+    log(f"bench: {args.form}: {args.steps} steps in {ev_s * 1e3:.1f} ms (events) / {wall * 1e3:.1f} ms (host) "
+        f"-> {value:.4e} TP-steps/s")
+    dE = float(np.max(np.abs((E1 - E0) / E0)))
+
+    # profiling pass: work counters and per-kernel event timing (no graph)
+    roof = None
+    extra = {}
+    if form == mint.FORM_BATCHED:
+        gp = make_ctx(form, guess, scorer, stats=3, block_size=shape[0], tps_per_lane=shape[1],
+                      chunk_steps=shape[2])
+        nprof = min(args.steps, 256)
+        gp.upload(st)
+        gp.reset_counters()
+        gp.step(nprof)
+        s = gp.stats()
+        t = gp.timing()
+        main_steps = s["tp_steps"] - s["enc_tp_steps"]
+        nbar = s["newton_iters"] / max(s["kepler_solves"], 1)
+        F = flops_per_tp_step(nbar)
+        avg_main_ms = t["ms_tp_main"] / max(t["n_tp_main"], 1)
+        achieved = F * main_steps / (t["ms_tp_main"] / 1e3)
+        roof = {"bound": "fp64", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK, "traffic": None,
+                "kernel": "tp_main_kernel", "flop_per_tp_step": F, "newton_mean": nbar,
+                "tp_steps_per_launch": main_steps / max(t["n_tp_main"], 1), "avg_launch_ms": avg_main_ms}
+        extra["profile"] = {"steps": nprof, "stats": s, "timing": t,
+                            "bs_fraction": s["bs_steps"] / max(s["tp_steps"], 1)}
+        gp.close()
+
+    forms = {args.form: value}
+    if not args.no_ablation and form == mint.FORM_BATCHED:
+        # flat: one launch per step over all systems
+        gf = make_ctx(mint.FORM_FLAT, mint.GUESS_SERIES, 0)
+        fs = min(args.steps, 200)
+        _, fe, _, _ = timed_run(gf, st, fs, 2)
+        forms["flat"] = ntp * fs / fe
+        gf.close()
+        # plain: the drop-in used like the reference uses its kernel (4 systems
+        # per launch, a launch per step, crude Kepler guess); timed on a sample
+        ps = st.subset(np.arange(min(args.plain_systems, st.n_systems)))
+        gpl = make_ctx(mint.FORM_PLAIN, mint.GUESS_CRUDE, 0)
+        _, pe, _, _ = timed_run(gpl, ps, args.plain_steps, 2)
+        forms["plain"] = ps.n_tp * args.plain_steps / pe
+        extra["plain_sample"] = f"systems 0-{ps.n_systems - 1} ({ps.n_tp} TPs), {args.plain_steps} steps"
+        gpl.close()
+        log(f"bench: forms {forms}")
+
+    base = None if args.no_baseline else cpu_baseline(st)
+    if base:
+        log(f"bench: cpu baseline {base}")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TP-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ev_s * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "p9_sweep_4096", "systems": st.n_systems, "tp_total": ntp, "massive_per_system": 5,
+                   "dt_yr": 0.5, "form": args.form, "kepler_guess": int(guess), "scorer": scorer,
+                   "launch_shape": shape},
+        "roofline": roof,
+        "cpu_baseline": base,
+        "forms_tp_steps_per_s": forms,
+        "speedup_vs_plain": value / forms["plain"] if "plain" in forms else None,
+        "energy": {"max_rel_dE_over_run": dE, "status_bits_or": int(np.bitwise_or.reduce(status))},
+        "host_wall_s": wall,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,7 @@ typedef struct mint_stats {
   int64_t deferred;        /* batched: TP-steps handed to the encounter queue */
   int64_t prebucketed;     /* batched: TPs routed to the queue by the scorer */
   int64_t idle_lane_steps; /* batched: lane-steps left idle in the main kernel */
+  int64_t enc_tp_steps;    /* batched: TP steps taken by the encounter-queue kernel */
 } mint_stats;
 
 /* Per-kernel timings from HIP events on the context stream (ms, launches). */

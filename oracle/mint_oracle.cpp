@@ -12,9 +12,9 @@
 // built with -ffp-contract=off), so the two agree to the last bit.
 //
 // Pinning (see tests/test_oracle.py): Stumpff series vs closed forms, Kepler
-// drift vs a brute-force Kepler-equation solve in long double, the Lagrange
-// identity f*gdot - fdot*g = 1, energy and angular momentum of pure Kepler
-// drifts, changeover identities K(0)=0, K(1)=1, K(y)+K(1-y)=1, BS on pure
+// drift vs a 40-digit universal-variable solve (mpmath root finder, closed-form
+// Stumpff functions, Lagrange identity f*gdot - fdot*g = 1 checked there),
+// energy and angular momentum of pure Kepler drifts, changeover identities K(0)=0, K(1)=1, K(y)+K(1-y)=1, BS on pure
 // Kepler vs the analytic drift, second-order energy convergence of the whole
 // step, and the hand-built encounter threshold cases.
 #include <cmath>
@@ -105,6 +105,8 @@ int kepler_drift(Body &b, double mu, double dt, int guess, bool &ok, double *s_r
   double G0 = 0, G1 = 0, G2 = 0, G3 = 0, ds = 0;
   int it = 0;
   ok = false;
+  // bracket from the monotonicity of F (F' = r > 0, F(0) = -dt)
+  double lo = dt >= 0.0 ? 0.0 : -INFINITY, hi = dt >= 0.0 ? INFINITY : 0.0;
   for (;;) {
     ++it;
     double c[4];
@@ -124,7 +126,15 @@ int kepler_drift(Body &b, double mu, double dt, int guess, bool &ok, double *s_r
       break;
     }
     if (it >= NEWTON_MAXIT) break;
-    s -= ds;
+    if (F < 0.0) lo = s; else hi = s;
+    double next = s - ds;
+    if (!(next > lo && next < hi)) {
+      // Laguerre-Conway step with n = 5, then bisection
+      const double disc = std::fabs(std::fma(16.0 * rp, rp, -(20.0 * F * rpp)));
+      next = s - (5.0 * F) / (rp + std::copysign(std::sqrt(disc), rp));
+      if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+    }
+    s = next;
   }
   if (s_root) *s_root = s - ds;
   // carry G1..G3 from s to s - ds (G_k' = G_{k-1}, G_0' = -beta G_1)

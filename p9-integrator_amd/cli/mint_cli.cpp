@@ -1,0 +1,289 @@
+// mint_cli.cpp — the `mint` command line tool on top of libmint.
+//
+//   mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]
+//
+// Same flags and JSON schema as the reference driver (README.md:10-19;
+// args_parse src/simutils.cuh:138-175; sim_from_config_file
+// src/simutils.cuh:232-301), with these documented differences:
+//   -t is an exact step count (default 1, as README.md:18 says); the
+//      reference counts 1e6-step launches (src/mercurius.cu:179).
+//   every sweep in the file runs on the selected device as its own system;
+//      a sweep replaces body 0 including its mass (the reference drops the
+//      sweep mass, src/mercurius.cu:126-164, and forces mean_anomaly to 0,
+//      src/simutils.cuh:299).
+//   optional keys: "dt" (default 0.8219, src/mercurius.cu:43), "n_hill",
+//      "bs_tol", "softening2", "central_mass", "form" (plain|flat|batched),
+//      "checkpoint_every" (steps between output checkpoints; default: end only).
+// Output files keep the reference formats: positions appended to output_<dev>
+// (src/simutils.cuh:186-230) and elements in elements_<dev>.txt
+// (src/mercurius.cu:210-248; src/simutils.cuh:177-184).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/mint.h"
+#include "json_min.h"
+
+namespace {
+
+struct Body {
+  std::string name;
+  double mass, a, e, inc, node, argp, manom;
+};
+
+Body body_from(const jsonmin::Value &v, const std::string &dflt_name) {
+  Body b;
+  b.name = v.string_or("name", dflt_name);
+  b.mass = v.at("mass").number();
+  b.a = v.at("semi_major_axis").number();
+  b.e = v.at("eccentricity").number();
+  b.inc = v.at("inclination").number();
+  b.node = v.at("longitude_of_ascending_node").number();
+  b.argp = v.at("argument_of_perihelion").number();
+  b.manom = v.number_or("mean_anomaly", 0.0);
+  return b;
+}
+
+struct Args {
+  std::string config, output;
+  long long steps = 1;
+  int device = 0;
+  bool info = false, print_positions = false;
+};
+
+int usage() {
+  std::cerr << "usage: mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]\n";
+  return 2;
+}
+
+bool parse_args(int argc, char **argv, Args &a) {
+  for (int i = 1; i < argc; ++i) {
+    const std::string f = argv[i];
+    auto need = [&](const char *what) -> const char * {
+      if (i + 1 >= argc) {
+        std::cerr << "mint: " << f << " needs " << what << "\n";
+        return nullptr;
+      }
+      return argv[++i];
+    };
+    if (f == "-i") {
+      a.info = true;
+    } else if (f == "-p") {
+      a.print_positions = true;
+    } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d") {
+      const char *v = need(f == "-c" ? "a file" : f == "-o" ? "a file" : "a number");
+      if (!v) return false;
+      if (f == "-c") a.config = v;
+      if (f == "-o") a.output = v;
+      if (f == "-t") a.steps = std::atoll(v);
+      if (f == "-d") a.device = std::atoi(v);
+    } else {
+      std::cerr << "mint: unknown flag " << f << "\n";
+      return false;
+    }
+  }
+  return !a.config.empty() && a.steps >= 0;
+}
+
+void check(int rc, void *ctx, const char *what) {
+  if (rc != 0) {
+    std::cerr << "mint: " << what << " failed (" << rc << "): " << mint_last_error(ctx) << "\n";
+    std::exit(1);
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Args args;
+  if (!parse_args(argc, argv, args)) return usage();
+
+  std::ifstream in(args.config);
+  if (!in) {
+    std::cerr << "mint: cannot open " << args.config << "\n";
+    return 1;
+  }
+  std::stringstream ss;
+  ss << in.rdbuf();
+  jsonmin::Value cfgj;
+  std::vector<Body> bodies, sweeps;
+  try {
+    cfgj = jsonmin::parse(ss.str());
+    const jsonmin::Value &bl = cfgj.at("bodies");
+    for (size_t i = 0; i < bl.arr.size(); ++i) bodies.push_back(body_from(bl.arr[i], "body-" + std::to_string(i)));
+    if (cfgj.has("sweeps"))
+      for (const auto &s : cfgj.at("sweeps").arr) sweeps.push_back(body_from(s, "sweep"));
+  } catch (const std::exception &e) {
+    std::cerr << "mint: " << args.config << ": " << e.what() << "\n";
+    return 1;
+  }
+  if (bodies.empty()) {
+    std::cerr << "mint: no bodies in " << args.config << "\n";
+    return 1;
+  }
+
+  mint_config cfg;
+  mint_config_default(&cfg);
+  cfg.dt = cfgj.number_or("dt", cfg.dt);
+  cfg.n_hill = cfgj.number_or("n_hill", cfg.n_hill);
+  cfg.bs_tol = cfgj.number_or("bs_tol", cfg.bs_tol);
+  cfg.softening2 = cfgj.number_or("softening2", cfg.softening2);
+  cfg.central_mass = cfgj.number_or("central_mass", cfg.central_mass);
+  const std::string form = cfgj.string_or("form", "batched");
+  cfg.form = form == "plain" ? MINT_FORM_PLAIN : form == "flat" ? MINT_FORM_FLAT : MINT_FORM_BATCHED;
+  cfg.device = args.device;
+  const long long ckpt = (long long)cfgj.number_or("checkpoint_every", (double)args.steps);
+
+  // systems: one per sweep (sweep replaces body 0), or the bodies as given
+  const int S = sweeps.empty() ? 1 : (int)sweeps.size();
+  const int NB = (int)bodies.size();
+  // massive bodies (mass > 1e-12, src/mercurius.cu:9) first, original order kept
+  std::vector<int> order;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < NB; ++i) {
+      const double m = (i == 0 && !sweeps.empty()) ? sweeps[0].mass : bodies[i].mass;
+      if ((m > 1e-12) == (pass == 0)) order.push_back(i);
+    }
+  int NM = 0;
+  for (int i = 0; i < NB; ++i) {
+    const double m = (i == 0 && !sweeps.empty()) ? sweeps[0].mass : bodies[i].mass;
+    NM += m > 1e-12;
+  }
+  const int NT = NB - NM;
+  if (NM < 1 || NM > 8) {
+    std::cerr << "mint: " << NM << " massive bodies; this build integrates 1 to 8 massive bodies per system\n";
+    return 1;
+  }
+  for (const Body &s : sweeps)
+    if (!(s.mass > 1e-12)) {
+      std::cerr << "mint: sweep bodies must be massive\n";
+      return 1;
+    }
+
+  const double mu = cfg.central_mass;
+  std::vector<int64_t> off(S + 1);
+  std::vector<double> m((size_t)S * NM), mx((size_t)S * NM * 3), mv((size_t)S * NM * 3);
+  std::vector<double> t[6];
+  for (auto &a : t) a.resize((size_t)S * NT);
+  for (int s = 0; s < S; ++s) {
+    off[s] = (int64_t)s * NT;
+    for (int k = 0; k < NB; ++k) {
+      const int i = order[k];
+      const Body &b = (i == 0 && !sweeps.empty()) ? sweeps[s] : bodies[i];
+      double x[3], v[3];
+      check(mint_elements_to_cartesian(1, mu, &b.inc, &b.node, &b.argp, &b.manom, &b.e, &b.a, x, v), nullptr,
+            "elements_to_cartesian");
+      if (k < NM) {
+        m[(size_t)s * NM + k] = b.mass;
+        for (int d = 0; d < 3; ++d) {
+          mx[((size_t)s * NM + k) * 3 + d] = x[d];
+          mv[((size_t)s * NM + k) * 3 + d] = v[d];
+        }
+      } else {
+        const size_t j = (size_t)s * NT + (k - NM);
+        for (int d = 0; d < 3; ++d) {
+          t[d][j] = x[d];
+          t[3 + d][j] = v[d];
+        }
+      }
+    }
+  }
+  off[S] = (int64_t)S * NT;
+
+  void *ctx = nullptr;
+  if (mint_create(&cfg, &ctx) != 0) {
+    std::cerr << "mint: " << mint_last_error(nullptr) << "\n";
+    return 1;
+  }
+  mint_state st{S, NM, off.data(), m.data(), mx.data(), mv.data(), t[0].data(), t[1].data(), t[2].data(),
+                t[3].data(), t[4].data(), t[5].data(), nullptr};
+  check(mint_upload(ctx, &st), ctx, "upload");
+  std::vector<double> E0(S), E1(S);
+  check(mint_energy(ctx, E0.data()), ctx, "energy");
+  if (args.info) {
+    std::cout << "Simulation: " << S << " system(s), " << NM << " massive and " << NT << " massless bodies each, dt "
+              << cfg.dt << " yr, " << args.steps << " steps, form " << form << "\n";
+    for (int k = 0; k < NB; ++k) {
+      const Body &b = bodies[order[k]];
+      std::cout << "Body " << order[k] << " " << b.name << ": mass " << b.mass << " a " << b.a << " e " << b.e
+                << " i " << b.inc << "\n";
+    }
+  }
+  if (cfg.form == MINT_FORM_BATCHED && (long long)S * NT >= 100000) {
+    int32_t best[3];
+    check(mint_autotune(ctx, nullptr, best), ctx, "autotune");
+    if (args.info) std::cout << "Launch shape: block " << best[0] << ", TPs/lane " << best[1] << ", window " << best[2] << "\n";
+  }
+
+  const std::string pos_file = args.output.empty() ? "output_" + std::to_string(args.device) : args.output;
+  long long done = 0;
+  std::vector<double> ox(mx.size()), ov(mv.size());
+  std::vector<double> o[6];
+  for (auto &a : o) a.resize(t[0].size());
+  mint_state out{S, NM, off.data(), nullptr, ox.data(), ov.data(), o[0].data(), o[1].data(), o[2].data(),
+                 o[3].data(), o[4].data(), o[5].data(), nullptr};
+  while (done < args.steps) {
+    const long long n = std::min(ckpt > 0 ? ckpt : args.steps, args.steps - done);
+    check(mint_step(ctx, n), ctx, "step");
+    done += n;
+    check(mint_download(ctx, &out), ctx, "download");
+    // positions (write_positions format, src/simutils.cuh:186-230)
+    std::ofstream pf(pos_file, std::ios::app);
+    if (!pf) {
+      std::cerr << "mint: cannot open " << pos_file << "\n";
+      return 1;
+    }
+    int nan = 0;
+    pf << "#\n";
+    std::vector<std::vector<double>> xyz((size_t)S * NB, std::vector<double>(6));
+    for (int s = 0; s < S; ++s) {
+      pf << "$\n";
+      for (int i = 0; i < NB; ++i) {
+        int k = 0;
+        while (order[k] != i) ++k;
+        double *q = xyz[(size_t)s * NB + i].data();
+        for (int d = 0; d < 3; ++d) {
+          q[d] = k < NM ? ox[((size_t)s * NM + k) * 3 + d] : o[d][(size_t)s * NT + (k - NM)];
+          q[3 + d] = k < NM ? ov[((size_t)s * NM + k) * 3 + d] : o[3 + d][(size_t)s * NT + (k - NM)];
+        }
+        if (std::isnan(q[0]) || std::isnan(q[1]) || std::isnan(q[2])) ++nan;
+        pf << q[0] << " " << q[1] << " " << q[2] << "\n";
+        if (args.print_positions) std::cout << s << " " << i << " " << q[0] << " " << q[1] << " " << q[2] << "\n";
+      }
+      pf << "\n";
+    }
+    pf << "\n";
+    std::cout << "NaN counter after step " << done << ": " << nan << std::endl;
+    // elements (src/mercurius.cu:210-248)
+    std::ofstream ef("elements_" + std::to_string(args.device) + ".txt");
+    ef << "Timestep: " << done << "\n";
+    for (int s = 0; s < S; ++s) {
+      ef << "Experiment: " << s << "\n";
+      for (int i = 0; i < NB; ++i) {
+        const double *q = xyz[(size_t)s * NB + i].data();
+        double el[6];
+        mint_cartesian_to_elements(1, mu, q, q + 3, &el[0], &el[1], &el[2], &el[3], &el[4], &el[5]);
+        ef << el[0] << " " << el[1] << " " << el[2] << " " << el[3] << " " << el[4] << " " << el[5] << "\n";
+      }
+      ef << "\n";
+    }
+  }
+  check(mint_energy(ctx, E1.data()), ctx, "energy");
+  std::vector<uint32_t> status(S);
+  check(mint_status(ctx, status.data()), ctx, "status");
+  double worst = 0.0;
+  uint32_t any = 0;
+  for (int s = 0; s < S; ++s) {
+    worst = std::max(worst, std::fabs((E1[s] - E0[s]) / E0[s]));
+    any |= status[s];
+  }
+  std::cout << "Energy drift max |dE/E| over systems: " << worst << "; status bits: " << any << std::endl;
+  mint_destroy(ctx);
+  return 0;
+}

@@ -299,7 +299,10 @@ int mint_upload(void *ctx, const mint_state *st) {
       return c->fail(MINT_EINVAL, "mint_upload: massive masses must exceed 1e-12 (src/mercurius.cu:9)");
   hipSetDevice(c->cfg.device);
   hipStreamSynchronize(c->stream);
-  c->free_device();
+  // same shape as the resident batch: keep the buffers (and captured graphs)
+  const bool reuse = c->uploaded && c->S == S && c->NM == NM && c->Ntp == N &&
+                     std::equal(c->tp_offset.begin(), c->tp_offset.end(), st->tp_offset);
+  if (!reuse) c->free_device();
   c->S = S;
   c->NM = NM;
   c->Ntp = N;
@@ -356,6 +359,7 @@ int mint_upload(void *ctx, const mint_state *st) {
 
   int rc;
   const size_t mb = (size_t)S * NM * 3 * sizeof(double);
+  if (!reuse) {
   if ((rc = c->dalloc((void **)&c->d_prm, prm.size() * sizeof(double), "params"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mx, mb, "massive x"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mv, mb, "massive v"))) return rc;
@@ -369,6 +373,7 @@ int mint_upload(void *ctx, const mint_state *st) {
   if ((rc = c->dalloc((void **)&c->d_stats, S_COUNT * sizeof(unsigned long long), "stats"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_kep, KM_NP * sizeof(float), "kepler mlp"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_sc, SC_NP * sizeof(float), "scorer"))) return rc;
+  }
 
   if ((rc = c->hip(hipMemcpy(c->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice), "H2D params"))) return rc;
   if ((rc = c->hip(hipMemcpy(c->d_mx, st->mx, mb, hipMemcpyHostToDevice), "H2D massive x"))) return rc;
@@ -393,6 +398,7 @@ int mint_upload(void *ctx, const mint_state *st) {
     hipMemcpy(c->d_nnw_sc, c->w_sc.data(), SC_NP * sizeof(float), hipMemcpyHostToDevice);
   else
     hipMemset(c->d_nnw_sc, 0, SC_NP * sizeof(float));
+  if ((rc = c->hip(hipDeviceSynchronize(), "upload"))) return rc;
   c->uploaded = true;
   c->timing = mint_timing{};
   return MINT_OK;
@@ -507,6 +513,7 @@ int mint_get_stats(void *ctx, mint_stats *out) {
   out->deferred = (int64_t)v[S_DEFER];
   out->prebucketed = (int64_t)v[S_PREBUCKET];
   out->idle_lane_steps = (int64_t)v[S_IDLE];
+  out->enc_tp_steps = (int64_t)v[S_ENCSTEPS];
   return MINT_OK;
 }
 

@@ -139,6 +139,8 @@ __device__ __forceinline__ double guess_series(const KepStart &k, double dt) {
 // stopping when the predicted post-update error |F''| ds^2 / (2 F') is below
 // 2^-52 |s| and |ds| <= 2e-6 |s|; the G-functions are then carried to s - ds by
 // a second-order Taylor step (G_k' = G_{k-1}) instead of a fresh evaluation.
+// A step that leaves the bracket kept from the signs of F is replaced by a
+// Laguerre-Conway step, or by bisection.
 // Returns the Newton iteration count; ok=false if the cap was hit.
 struct KepSol {
   double G0, G1, G2, G3;  // G-functions at s
@@ -152,6 +154,9 @@ __device__ __forceinline__ KepSol kepler_solve(const KepStart &k, double s, doub
   KepSol o;
   o.it = 0;
   o.ok = false;
+  // F is increasing (F' = r > 0) and F(0) = -dt, so the root lies in (lo, hi).
+  double lo = dt >= 0.0 ? 0.0 : -INFINITY;
+  double hi = dt >= 0.0 ? INFINITY : 0.0;
   while (true) {
     ++o.it;
     double c0, c1, c2, c3;
@@ -171,7 +176,20 @@ __device__ __forceinline__ KepSol kepler_solve(const KepStart &k, double s, doub
       break;
     }
     if (o.it >= NEWTON_MAXIT) break;
-    s -= o.ds;
+    if (F < 0.0)
+      lo = s;
+    else
+      hi = s;
+    double sn = s - o.ds;
+    if (!(sn > lo && sn < hi)) {
+      // Newton left the bracket: Laguerre-Conway step (n = 5, Conway 1986),
+      // then bisection if that also leaves it.
+      const double d2 = fabs(fma(16.0 * rp, rp, -(20.0 * F * rpp)));
+      const double den = rp + copysign(sqrt(d2), rp);
+      sn = s - (5.0 * F) / den;
+      if (!(sn > lo && sn < hi)) sn = 0.5 * (lo + hi);
+    }
+    s = sn;
   }
   o.s = s;
   return o;

@@ -46,7 +46,8 @@ enum {
   S_DEFER = 5,
   S_PREBUCKET = 6,
   S_IDLE = 7,
-  S_COUNT = 8
+  S_ENCSTEPS = 8,
+  S_COUNT = 9
 };
 
 __device__ __forceinline__ void q_put(QEntry *e, int tp, int sys, int k0, int phase, double x,
@@ -500,6 +501,7 @@ __global__ void __launch_bounds__(256) tp_encounter_kernel(
     wave_add(stats + S_NEWTON, n_newton);
     wave_add(stats + S_BSSTEPS, n_bs);
     wave_add(stats + S_BSRHS, n_rhs);
+    wave_add(stats + S_ENCSTEPS, n_step);
   }
 }
 

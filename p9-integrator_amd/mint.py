@@ -74,7 +74,8 @@ class _State(C.Structure):
 
 class _Stats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("tp_steps", "kepler_solves", "newton_iters", "bs_steps",
-                                         "bs_rhs_evals", "deferred", "prebucketed", "idle_lane_steps")]
+                                         "bs_rhs_evals", "deferred", "prebucketed", "idle_lane_steps",
+                                         "enc_tp_steps")]
 
 
 class _Timing(C.Structure):

@@ -1,0 +1,72 @@
+"""GPU parity where the Bulirsch-Stoer close-encounter path is exercised, and
+the batch-invariance properties of the batched form (bitwise)."""
+import numpy as np
+import pytest
+
+import mint
+import mint_workloads as W
+import oracle as O
+from gpu_cases import encounter_system, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_FLAT, mint.FORM_BATCHED])
+def test_encounter_cloud_matches_oracle(form):
+    st = encounter_system()
+    print(f"encounter cloud form={form}", flush=True)
+    kw = dict(chunk_steps=16, block_size=64, tps_per_lane=2) if form == mint.FORM_BATCHED else {}
+    out, status, s = run_gpu(st, 40, 0.5, form, mint.GUESS_SERIES, **kw)
+    ref, rst, rs = O.run(st, 40, 0.5, guess=1)
+    assert rs["bs_steps"] > 1000, rs               # the BS path is really taken
+    assert s["bs_steps"] == rs["bs_steps"]
+    assert s["bs_rhs_evals"] == rs["bs_rhs_evals"]
+    assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
+    assert np.array_equal(out.mx, ref["mx"])
+    assert np.all(status == rst)
+
+
+def test_system_alone_equals_system_in_batch():
+    st = W.p9_sweep(systems=np.arange(24), tp_scale=1 / 8)
+    out, _, _ = run_gpu(st, 64, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=32, block_size=256,
+                        tps_per_lane=1)
+    for s in (0, 7, 23):
+        one = st.subset([s])
+        o1, _, _ = run_gpu(one, 64, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=16, block_size=64,
+                           tps_per_lane=4)
+        a, b = st.tp_offset[s], st.tp_offset[s + 1]
+        assert np.array_equal(o1.tx, out.tx[a:b]) and np.array_equal(o1.tv, out.tv[a:b])
+        assert np.array_equal(o1.mx[0], out.mx[s])
+
+
+def test_permuting_tps_permutes_results():
+    st = W.p9_sweep(systems=[3], tp_scale=1 / 2)
+    perm = np.random.default_rng(5).permutation(st.n_tp)
+    sp = st.copy()
+    sp.tx = st.tx[perm]
+    sp.tv = st.tv[perm]
+    a, _, _ = run_gpu(st, 50, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=25, block_size=128,
+                      tps_per_lane=1)
+    b, _, _ = run_gpu(sp, 50, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=25, block_size=128,
+                      tps_per_lane=1)
+    assert np.array_equal(a.tx[perm], b.tx) and np.array_equal(a.tv[perm], b.tv)
+
+
+def test_run_twice_is_deterministic_and_launch_shape_invariant():
+    st = encounter_system(n_tp=300, seed=3)
+    a, _, _ = run_gpu(st, 30, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=8, block_size=64,
+                      tps_per_lane=1)
+    b, _, _ = run_gpu(st, 30, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=8, block_size=64,
+                      tps_per_lane=1)
+    c, _, _ = run_gpu(st, 30, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=30, block_size=512,
+                      tps_per_lane=4)
+    assert np.array_equal(a.tx, b.tx) and np.array_equal(a.tx, c.tx) and np.array_equal(a.tv, c.tv)
+
+
+def test_graph_replay_equals_direct_launches():
+    st = W.p9_sweep(systems=np.arange(6), tp_scale=1 / 4)
+    a, _, _ = run_gpu(st, 70, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=16, block_size=128,
+                      tps_per_lane=1, use_graph=1)
+    b, _, _ = run_gpu(st, 70, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=16, block_size=128,
+                      tps_per_lane=1, use_graph=0)
+    assert np.array_equal(a.tx, b.tx) and np.array_equal(a.tv, b.tv)
