@@ -643,7 +643,8 @@ int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const doub
     const double ul = std::atan2(dot3h(r, pvec), dot3h(r, nvec));
     double f;
     if (e > 1e-14) {
-      f = std::atan2(dot3h(ev, pvec) * 0.0 + (rv * hn) / mu, dot3h(h, h) / (mu * rn) - 1.0);
+      // e cos f = h^2/(mu r) - 1,  e sin f = h (r.v)/(mu r)
+      f = std::atan2((hn * rv) / (mu * rn), dot3h(h, h) / (mu * rn) - 1.0);
     } else {
       f = ul;
     }
