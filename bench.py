@@ -84,6 +84,21 @@ def cpu_baseline(st_full, budget_s=12.0):
                        f"{dt:.1f} s")
 
 
+def copy_probe(nbytes):
+    """One hipMemcpy device-to-device of nbytes (a plain copy kernel of known
+    size: FETCH_SIZE / WRITE_SIZE per byte of it calibrate the counters)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +111,13 @@ def main():
     ap.add_argument("--plain-systems", type=int, default=64, help="systems in the plain-form sample")
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "p9-integrator_amd", "autotune.cache"))
+    ap.add_argument("--shape", default=None, help="block,tps_per_lane,window: skip the autotuner")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (systems would shard "
                                    "across GPUs with no exchange, see the sharding paragraph of DESIGN.md)"}))
@@ -112,9 +133,12 @@ def main():
     guess = mint.GUESS_LEARNED if args.form == "learned" else mint.GUESS_SERIES
     scorer = 1 if args.form == "learned" else 0
 
-    g = make_ctx(form, guess, scorer)
-    shape = None
-    if form == mint.FORM_BATCHED:
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+    if shape:
+        g = make_ctx(form, guess, scorer, block_size=shape[0], tps_per_lane=shape[1], chunk_steps=shape[2])
+    else:
+        g = make_ctx(form, guess, scorer)
+    if form == mint.FORM_BATCHED and not shape:
         g.upload(st)
         shape = g.autotune(args.tune_cache)
         log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]}")

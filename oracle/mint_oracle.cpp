@@ -89,17 +89,20 @@ void stumpff(double z, double c[4]) {
 // src/sim.cuh:704-713) ---------------------------------------------------------
 int kepler_drift(Body &b, double mu, double dt, int guess, bool &ok, double *s_root = nullptr) {
   const double r0 = std::sqrt(dot(b.x, b.x));
+  const double inv_r0 = 1.0 / r0;
   const double v2 = dot(b.v, b.v);
   const double eta0 = dot(b.x, b.v);
-  const double beta = (2.0 * mu) / r0 - v2;
+  const double beta = std::fma(2.0 * mu, inv_r0, -v2);
   const double zeta0 = std::fma(-beta, r0, mu);
   double s;
   if (guess == 0) {
-    s = dt / r0;
+    s = dt * inv_r0;
   } else {
-    const double u = dt / r0;
-    const double a = -(eta0 / (2.0 * r0));
-    const double bb = (eta0 * eta0) / (2.0 * (r0 * r0)) - zeta0 / (6.0 * r0);
+    // s = u (1 + a u + b u^2): third-order inversion of the Kepler equation
+    const double u = dt * inv_r0;
+    const double e = eta0 * inv_r0;
+    const double a = -0.5 * e;
+    const double bb = std::fma(0.5 * e, e, -(zeta0 * inv_r0) * (1.0 / 6.0));
     s = u * std::fma(u, std::fma(u, bb, a), 1.0);
   }
   double G0 = 0, G1 = 0, G2 = 0, G3 = 0, ds = 0;
@@ -145,10 +148,11 @@ int kepler_drift(Body &b, double mu, double dt, int guess, bool &ok, double *s_r
   const double g2 = G2 + std::fma(G1, hs, t0);
   const double g3 = G3 + std::fma(G2, hs, t1);
   const double r = std::fma(eta0, g1, std::fma(zeta0, g2, r0));
-  const double fm1 = -((mu * g2) / r0);
-  const double g = dt - mu * g3;
-  const double fdot = -((mu * g1) / (r0 * r));
-  const double gm1 = -((mu * g2) / r);
+  const double inv_r = 1.0 / r;
+  const double fm1 = -(mu * g2) * inv_r0;
+  const double g = std::fma(-mu, g3, dt);
+  const double fdot = -(mu * g1) * (inv_r0 * inv_r);
+  const double gm1 = -(mu * g2) * inv_r;
   const V3 x = b.x, v = b.v;
   b.x.x = x.x + std::fma(fm1, x.x, g * v.x);
   b.x.y = x.y + std::fma(fm1, x.y, g * v.y);

@@ -55,6 +55,7 @@ void Ctx::free_device() {
   d_prm = d_mx = d_mv = d_xs0 = d_rec = nullptr;
   for (auto &p : d_tp) p = nullptr;
   d_tp_sys = nullptr;
+  d_pre = nullptr;
   d_tiles = nullptr;
   d_q = nullptr;
   d_qctl = nullptr;
@@ -367,6 +368,7 @@ int mint_upload(void *ctx, const mint_state *st) {
   for (int d = 0; d < 6; ++d)
     if ((rc = c->dalloc((void **)&c->d_tp[d], (size_t)N * sizeof(double), "TP state"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_tp_sys, (size_t)N * sizeof(int), "TP system index"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_pre, (size_t)N, "pre-bucket flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_q, (size_t)std::max<int64_t>(N, 1) * sizeof(QEntry), "BS queue"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_qctl, 4 * sizeof(unsigned int), "queue counters"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_status, (size_t)S * sizeof(unsigned int), "status"))) return rc;

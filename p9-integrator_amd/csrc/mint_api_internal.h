@@ -36,6 +36,7 @@ struct Ctx {
   double *d_prm = nullptr, *d_mx = nullptr, *d_mv = nullptr, *d_xs0 = nullptr, *d_rec = nullptr;
   double *d_tp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int *d_tp_sys = nullptr;
+  unsigned char *d_pre = nullptr;  // per-TP pre-bucket flag of the current window
   Tile *d_tiles = nullptr;
   int ntiles = 0;
   long long tiles_key = -1;

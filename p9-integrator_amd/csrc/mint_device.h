@@ -45,12 +45,18 @@ __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; 
 //   [6, 6+3NM)       Y: massive positions after jump 1     (encounter test, BS)
 //   [6+3NM, 6+6NM)   W: massive velocities at drift start  (BS perturber drift)
 //   [6+6NM, 6+9NM)   Xe: massive positions after jump 2    (closing kick)
-__host__ __device__ constexpr int rec_size(int nm) { return ((6 + 9 * nm) + 1) & ~1; }
+//   [6+9NM, 6+21NM)  P4: perturber positions Kepler-drifted from (Y, W) by
+//                    tau = h/4, 2(h/4), 3(h/4) and h, the substep times of the
+//                    first two BS rows; a BS right-hand side at one of these
+//                    times reads them instead of solving Kepler's equation
+//                    (same function of the same inputs, so the same bits)
+__host__ __device__ constexpr int rec_size(int nm) { return ((6 + 21 * nm) + 1) & ~1; }
 __host__ __device__ constexpr int R_Q1(int) { return 0; }
 __host__ __device__ constexpr int R_Q2(int) { return 3; }
 __host__ __device__ constexpr int R_Y(int) { return 6; }
 __host__ __device__ constexpr int R_W(int nm) { return 6 + 3 * nm; }
 __host__ __device__ constexpr int R_XE(int nm) { return 6 + 6 * nm; }
+__host__ __device__ constexpr int R_P4(int nm) { return 6 + 9 * nm; }
 
 struct StepConst {
   double h;     // step
@@ -109,27 +115,30 @@ __device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double
 
 // Quantities of the universal-variable Kepler problem at the drift start.
 struct KepStart {
-  double r0, eta0, beta, zeta0;
+  double r0, ir0, eta0, beta, zeta0;  // ir0 = 1/r0
 };
 
 __device__ __forceinline__ KepStart kep_start(double x, double y, double z, double vx, double vy,
                                               double vz, double mu) {
   KepStart k;
   k.r0 = sqrt(dot3(x, y, z, x, y, z));
+  k.ir0 = 1.0 / k.r0;
   const double v2 = dot3(vx, vy, vz, vx, vy, vz);
   k.eta0 = dot3(x, y, z, vx, vy, vz);
-  k.beta = (2.0 * mu) / k.r0 - v2;
+  k.beta = fma(2.0 * mu, k.ir0, -v2);
   k.zeta0 = fma(-k.beta, k.r0, mu);
   return k;
 }
 
 // Starting guesses for the universal variable s.
-__device__ __forceinline__ double guess_crude(const KepStart &k, double dt) { return dt / k.r0; }
-// Third-order series inversion of dt = r0 s + eta0 s^2/2 + zeta0 s^3/6 + O(s^4).
+__device__ __forceinline__ double guess_crude(const KepStart &k, double dt) { return dt * k.ir0; }
+// Third-order series inversion of dt = r0 s + eta0 s^2/2 + zeta0 s^3/6 + O(s^4):
+// s = u (1 + a u + b u^2), u = dt/r0, a = -eta0/(2 r0), b = eta0^2/(2 r0^2) - zeta0/(6 r0).
 __device__ __forceinline__ double guess_series(const KepStart &k, double dt) {
-  const double u = dt / k.r0;
-  const double a = -(k.eta0 / (2.0 * k.r0));
-  const double b = (k.eta0 * k.eta0) / (2.0 * (k.r0 * k.r0)) - k.zeta0 / (6.0 * k.r0);
+  const double u = dt * k.ir0;
+  const double e = k.eta0 * k.ir0;
+  const double a = -0.5 * e;
+  const double b = fma(0.5 * e, e, -(k.zeta0 * k.ir0) * (1.0 / 6.0));
   return u * fma(u, fma(u, b, a), 1.0);
 }
 
@@ -208,10 +217,11 @@ __device__ __forceinline__ int kepler_drift_from(const KepStart &k, double s, do
   const double G2n = o.G2 + fma(o.G1, hs, t0);
   const double G3n = o.G3 + fma(o.G2, hs, t1);
   const double r = fma(k.eta0, G1n, fma(k.zeta0, G2n, k.r0));
-  const double fm1 = -((mu * G2n) / k.r0);
-  const double g = dt - mu * G3n;
-  const double fdot = -((mu * G1n) / (k.r0 * r));
-  const double gm1 = -((mu * G2n) / r);
+  const double ir = 1.0 / r;
+  const double fm1 = -(mu * G2n) * k.ir0;
+  const double g = fma(-mu, G3n, dt);
+  const double fdot = -(mu * G1n) * (k.ir0 * ir);
+  const double gm1 = -(mu * G2n) * ir;
   const double nx = x + fma(fm1, x, g * vx);
   const double ny = y + fma(fm1, y, g * vy);
   const double nz = z + fma(fm1, z, g * vz);
@@ -320,6 +330,7 @@ __device__ __forceinline__ bool encounter_tp(double x, double y, double z,
 template <int NM>
 __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const double *__restrict__ Y,
                                        const double *__restrict__ W,
+                                       const double *__restrict__ P4, double h,
                                        const double *__restrict__ prm, double mu, double eps2,
                                        double *f) {
   const double x = yv[0], y = yv[1], z = yv[2];
@@ -330,10 +341,24 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
   const double r = sqrt(r2);
   const double w0 = mu / (r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
+  const double hs4 = h / 4.0;
+  int ci = -1;
+  if (tau == hs4)
+    ci = 0;
+  else if (tau == 2.0 * hs4)
+    ci = 1;
+  else if (tau == 3.0 * hs4)
+    ci = 2;
+  else if (tau == h)
+    ci = 3;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     double px = Y[3 * j + 0], py = Y[3 * j + 1], pz = Y[3 * j + 2];
-    if (tau != 0.0) {
+    if (ci >= 0) {
+      px = P4[ci * 3 * NM + 3 * j + 0];
+      py = P4[ci * 3 * NM + 3 * j + 1];
+      pz = P4[ci * 3 * NM + 3 * j + 2];
+    } else if (tau != 0.0) {
       double qx = W[3 * j + 0], qy = W[3 * j + 1], qz = W[3 * j + 2];
       bool ok;
       kepler_drift_series(px, py, pz, qx, qy, qz, mu, tau, ok);
@@ -359,6 +384,7 @@ template <int NM>
 __device__ __noinline__ void gragg_tp(const double *y0, const double *f0, double t0, double H,
                                       int N, const double *__restrict__ Y,
                                       const double *__restrict__ W,
+                                      const double *__restrict__ P4, double h,
                                       const double *__restrict__ prm, double mu, double eps2,
                                       double *out, int &nrhs) {
   const double hs = H / (double)N;
@@ -370,7 +396,7 @@ __device__ __noinline__ void gragg_tp(const double *y0, const double *f0, double
     z1[i] = fma(hs, f0[i], y0[i]);
   }
   for (int m = 1; m < N; ++m) {
-    rhs_tp<NM>(t0 + (double)m * hs, z1, Y, W, prm, mu, eps2, f);
+    rhs_tp<NM>(t0 + (double)m * hs, z1, Y, W, P4, h, prm, mu, eps2, f);
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const double z2 = fma(h2, f[i], z0[i]);
@@ -378,7 +404,7 @@ __device__ __noinline__ void gragg_tp(const double *y0, const double *f0, double
       z1[i] = z2;
     }
   }
-  rhs_tp<NM>(t0 + H, z1, Y, W, prm, mu, eps2, f);
+  rhs_tp<NM>(t0 + H, z1, Y, W, P4, h, prm, mu, eps2, f);
   nrhs += N;
 #pragma unroll
   for (int i = 0; i < 6; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
@@ -396,6 +422,7 @@ __device__ __forceinline__ double bs_fac(int j) {
 template <int NM>
 __device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
                                              const double *__restrict__ W,
+                                             const double *__restrict__ P4,
                                              const double *__restrict__ prm, double mu,
                                              double eps2, double tol, int &nrhs) {
   uint32_t st = 0;
@@ -404,7 +431,7 @@ __device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double 
   while (t < h) {
     if (H > h - t) H = h - t;
     double f0[6];
-    rhs_tp<NM>(t, yv, Y, W, prm, mu, eps2, f0);
+    rhs_tp<NM>(t, yv, Y, W, P4, h, prm, mu, eps2, f0);
     nrhs += 1;
     const double xs = sqrt(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
     const double vs = sqrt(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
@@ -412,7 +439,7 @@ __device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double 
     double cur[6];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
-      gragg_tp<NM>(yv, f0, t, H, 2 << k, Y, W, prm, mu, eps2, cur, nrhs);
+      gragg_tp<NM>(yv, f0, t, H, 2 << k, Y, W, P4, h, prm, mu, eps2, cur, nrhs);
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
         for (int i = 0; i < 6; ++i) {

@@ -159,8 +159,34 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
       r[R_Y(NM) + i] = X[i];
       r[R_W(NM) + i] = V[i];
     }
+    // perturber positions at the substep times of the first BS rows
+    {
+      const double hs4 = sc.h / 4.0;
+      for (int q = 1; q <= 3; ++q) {
+        const double tau = 0.0 + (double)q * hs4;
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+          double px = X[3 * j], py = X[3 * j + 1], pz = X[3 * j + 2];
+          double qx = V[3 * j], qy = V[3 * j + 1], qz = V[3 * j + 2];
+          bool ok;
+          kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, tau, ok);
+          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 0] = px;
+          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 1] = py;
+          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 2] = pz;
+        }
+      }
+    }
     if (encounter_massive<NM>(X, prm)) {
       st |= ST_MASSIVE_ENC;
+      for (int j = 0; j < NM; ++j) {
+        double px = X[3 * j], py = X[3 * j + 1], pz = X[3 * j + 2];
+        double qx = V[3 * j], qy = V[3 * j + 1], qz = V[3 * j + 2];
+        bool ok;
+        kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, sc.h, ok);
+        r[R_P4(NM) + 9 * NM + 3 * j + 0] = px;
+        r[R_P4(NM) + 9 * NM + 3 * j + 1] = py;
+        r[R_P4(NM) + 9 * NM + 3 * j + 2] = pz;
+      }
       double sv[6 * NM];
       for (int j = 0; j < NM; ++j)
         for (int c = 0; c < 3; ++c) {
@@ -180,6 +206,9 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
         kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
                             V[3 * j + 2], sc.mu, sc.h, ok);
         if (!ok) st |= ST_NEWTON;
+        r[R_P4(NM) + 9 * NM + 3 * j + 0] = X[3 * j];
+        r[R_P4(NM) + 9 * NM + 3 * j + 1] = X[3 * j + 1];
+        r[R_P4(NM) + 9 * NM + 3 * j + 2] = X[3 * j + 2];
       }
     }
     momentum_q<NM>(V, prm, M, Q);
@@ -243,7 +272,7 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
     if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
       double yv[6] = {x, y, z, vx, vy, vz};
       int nr = 0;
-      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
       x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
       n_bs = 1;
       n_rhs = (unsigned)nr;
@@ -301,11 +330,11 @@ __device__ __forceinline__ void scorer_features(double x, double y, double z, do
 }
 
 template <int NM, int P, int GUESS>
-__global__ void __launch_bounds__(1024) tp_main_kernel(
+__global__ void __launch_bounds__(512) tp_main_kernel(
     const Tile *__restrict__ tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
     const double *__restrict__ prm_all, const double *__restrict__ xs0,
     const double *__restrict__ rec, int C, StepConst sc, const float *__restrict__ nnw,
-    const float *__restrict__ scw, float score_thresh, QEntry *__restrict__ q,
+    const unsigned char *__restrict__ pre, QEntry *__restrict__ q,
     unsigned int *__restrict__ qcount, unsigned int *__restrict__ status,
     unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
@@ -323,7 +352,7 @@ __global__ void __launch_bounds__(1024) tp_main_kernel(
   double X[P][3], V[P][3], A[P][3];
   bool act[P];
   long long idx[P];
-  unsigned int n_kep = 0, n_newton = 0, n_defer = 0, n_pre = 0, n_idle = 0, n_step = 0;
+  unsigned int n_kep = 0, n_newton = 0, n_defer = 0, n_idle = 0, n_step = 0;
   uint32_t st = 0;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
@@ -335,32 +364,14 @@ __global__ void __launch_bounds__(1024) tp_main_kernel(
       V[p][0] = tp.vx[idx[p]]; V[p][1] = tp.vy[idx[p]]; V[p][2] = tp.vz[idx[p]];
     }
   }
-  if (scw != nullptr) {
-    // Pre-bucket TPs the scorer expects to meet a changeover zone in this
-    // window; the encounter kernel then runs them from step 0.  The exact
-    // distance check still decides every step, so results do not change.
-    const double T = sc.h * (double)C;
+  if (pre != nullptr) {
+    // TPs the scorer pre-bucketed for this window start in the encounter queue
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      if (!act[p]) continue;
-      float best = -1e30f;
-      for (int j = 0; j < NM; ++j) {
-        float f[SC_IN];
-        const double rc = sqrt(prm[P_RC2(NM) + j]);
-        scorer_features(X[p][0], X[p][1], X[p][2], V[p][0], V[p][1], V[p][2], xsys[3 * j],
-                        xsys[3 * j + 1], xsys[3 * j + 2], rsys[R_W(NM) + 3 * j],
-                        rsys[R_W(NM) + 3 * j + 1], rsys[R_W(NM) + 3 * j + 2], rc, T, f);
-        best = fmaxf(best, scorer_fwd(scw, f));
-      }
-      if (best > score_thresh) {
-        const unsigned int slot = atomicAdd(qcount, 1u);
-        q_put(q + slot, (int)idx[p], s, 0, 0, X[p][0], X[p][1], X[p][2], V[p][0], V[p][1],
-              V[p][2]);
+    for (int p = 0; p < P; ++p)
+      if (act[p] && pre[idx[p]]) {
         act[p] = false;
-        ++n_pre;
         n_idle += (unsigned)C;
       }
-    }
   }
 #pragma unroll
   for (int p = 0; p < P; ++p)
@@ -414,9 +425,49 @@ __global__ void __launch_bounds__(1024) tp_main_kernel(
     wave_add(stats + S_KEPLER, n_kep);
     wave_add(stats + S_NEWTON, n_newton);
     wave_add(stats + S_DEFER, n_defer);
-    wave_add(stats + S_PREBUCKET, n_pre);
     wave_add(stats + S_IDLE, n_idle);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Learned pre-bucketing: one lane per TP.  A TP whose largest pair logit from
+// the scorer exceeds the threshold starts the window in the encounter queue
+// (phase 0) and is skipped by the main kernel.  The exact distance check
+// still decides every step inside the queue, so results do not change.
+template <int NM>
+__global__ void __launch_bounds__(256) tp_prebucket_kernel(
+    long long n, const int *__restrict__ tp_sys, TPArrays tp, const double *__restrict__ prm_all,
+    const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
+    const float *__restrict__ scw, float thresh, unsigned char *__restrict__ pre,
+    QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
+    unsigned long long *__restrict__ stats) {
+  constexpr int REC = rec_size(NM);
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int n_pre = 0;
+  if (i < n) {
+    const int s = tp_sys[i];
+    const double *prm = prm_all + (size_t)s * prm_size(NM);
+    const double *xsys = xs0 + (size_t)s * 3 * NM;
+    const double *r0 = rec + (size_t)s * C * REC;
+    const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
+    const double T = sc.h * (double)C;
+    float best = -1e30f;
+    for (int j = 0; j < NM; ++j) {
+      float f[SC_IN];
+      scorer_features(x, y, z, vx, vy, vz, xsys[3 * j], xsys[3 * j + 1], xsys[3 * j + 2],
+                      r0[R_W(NM) + 3 * j], r0[R_W(NM) + 3 * j + 1], r0[R_W(NM) + 3 * j + 2],
+                      sqrt(prm[P_RC2(NM) + j]), T, f);
+      best = fmaxf(best, scorer_fwd(scw, f));
+    }
+    const bool go = best > thresh;
+    pre[i] = go ? 1 : 0;
+    if (go) {
+      const unsigned int slot = atomicAdd(qcount, 1u);
+      q_put(q + slot, (int)i, s, 0, 0, x, y, z, vx, vy, vz);
+      n_pre = 1;
+    }
+  }
+  if (stats) wave_add(stats + S_PREBUCKET, n_pre);
 }
 
 // ---------------------------------------------------------------------------
@@ -463,7 +514,7 @@ __global__ void __launch_bounds__(256) tp_encounter_kernel(
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
-        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
         ++n_bs;
         n_rhs += (unsigned)nr;

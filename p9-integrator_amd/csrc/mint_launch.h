@@ -18,7 +18,7 @@ struct Launchers {
   hipError_t (*tp_main)(hipStream_t st, int guess, int P, int block, int nblocks,
                         const Tile *tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
                         const double *prm, const double *xs0, const double *rec, int C,
-                        StepConst sc, const float *nnw, const float *scw, float thresh,
+                        StepConst sc, const float *nnw, const unsigned char *pre,
                         QEntry *q, unsigned int *qcount, unsigned int *status,
                         unsigned long long *stats);
   hipError_t (*tp_encounter)(hipStream_t st, int guess, int grid, int block, const QEntry *q,
@@ -26,6 +26,10 @@ struct Launchers {
                              const double *prm, const double *xs0, const double *rec, int C,
                              StepConst sc, const float *nnw, unsigned int *status,
                              unsigned long long *stats);
+  hipError_t (*tp_prebucket)(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
+                             const double *prm, const double *xs0, const double *rec, int C,
+                             StepConst sc, const float *scw, float thresh, unsigned char *pre,
+                             QEntry *q, unsigned int *qcount, unsigned long long *stats);
 };
 
 const Launchers *launchers_nm1();

@@ -50,25 +50,25 @@ hipError_t tp_step_l(hipStream_t st, int guess, int block, long long i0, long lo
 template <int P, int G>
 hipError_t main_pg(hipStream_t st, int block, int nblocks, const Tile *tiles, int ntiles,
                    int tiles_per_xcd, TPArrays tp, const double *prm, const double *xs0,
-                   const double *rec, int C, StepConst sc, const float *nnw, const float *scw,
-                   float thresh, QEntry *q, unsigned int *qcount, unsigned int *status,
-                   unsigned long long *stats) {
+                   const double *rec, int C, StepConst sc, const float *nnw,
+                   const unsigned char *pre, QEntry *q, unsigned int *qcount,
+                   unsigned int *status, unsigned long long *stats) {
   tp_main_kernel<NMV, P, G><<<nblocks, block, 0, st>>>(tiles, ntiles, tiles_per_xcd, tp, prm, xs0,
-                                                       rec, C, sc, nnw, scw, thresh, q, qcount,
-                                                       status, stats);
+                                                       rec, C, sc, nnw, pre, q, qcount, status,
+                                                       stats);
   return hipGetLastError();
 }
 
 hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, const Tile *tiles,
                      int ntiles, int tiles_per_xcd, TPArrays tp, const double *prm,
                      const double *xs0, const double *rec, int C, StepConst sc, const float *nnw,
-                     const float *scw, float thresh, QEntry *q, unsigned int *qcount,
+                     const unsigned char *pre, QEntry *q, unsigned int *qcount,
                      unsigned int *status, unsigned long long *stats) {
-  if (block <= 0 || block > 1024 || (block & 63)) return hipErrorInvalidValue;
+  if (block <= 0 || block > 512 || (block & 63)) return hipErrorInvalidValue;
 #define MINT_MAIN(PP, GG)                                                                       \
   if (P == PP && guess == GG)                                                                   \
     return main_pg<PP, GG>(st, block, nblocks, tiles, ntiles, tiles_per_xcd, tp, prm, xs0, rec, \
-                           C, sc, nnw, scw, thresh, q, qcount, status, stats);
+                           C, sc, nnw, pre, q, qcount, status, stats);
   MINT_MAIN(1, 0) MINT_MAIN(1, 1) MINT_MAIN(1, 2)
   MINT_MAIN(2, 0) MINT_MAIN(2, 1) MINT_MAIN(2, 2)
   MINT_MAIN(4, 0) MINT_MAIN(4, 1) MINT_MAIN(4, 2)
@@ -101,12 +101,22 @@ hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const 
   return hipGetLastError();
 }
 
+hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
+                          const double *prm, const double *xs0, const double *rec, int C,
+                          StepConst sc, const float *scw, float thresh, unsigned char *pre,
+                          QEntry *q, unsigned int *qcount, unsigned long long *stats) {
+  if (n <= 0) return hipSuccess;
+  tp_prebucket_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, pre, q, qcount, stats);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 #define MINT_CAT2(a, b) a##b
 #define MINT_CAT(a, b) MINT_CAT2(a, b)
 const Launchers *MINT_CAT(launchers_nm, MINT_NM)() {
-  static const Launchers L = {ephemeris_l, tp_step_l, tp_main_l, tp_encounter_l};
+  static const Launchers L = {ephemeris_l, tp_step_l, tp_main_l, tp_encounter_l, tp_prebucket_l};
   return &L;
 }
 

@@ -91,11 +91,17 @@ int launch_window(Ctx *c, const Shape &sh, int C, KTimer *kt) {
   if (kt) kt->stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
   zero_u32_kernel<<<1, 64, 0, c->stream>>>(c->d_qctl, 4);
   if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
+  if (scw && c->Ntp > 0) {
+    rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, c->d_xs0, c->d_rec, C, sc,
+                                   scw, c->score_thresh, c->d_pre, c->d_q, c->d_qctl, stats),
+                "prebucket launch");
+    if (rc) return rc;
+  }
   if (c->ntiles > 0) {
     if (kt) kt->start();
     rc = c->hip(c->L->tp_main(c->stream, c->cfg.kepler_guess, sh.P, sh.block, nblocks, c->d_tiles,
-                              c->ntiles, tpx, tp, c->d_prm, c->d_xs0, c->d_rec, C, sc, nnw, scw,
-                              c->score_thresh, c->d_q, c->d_qctl, c->d_status, stats),
+                              c->ntiles, tpx, tp, c->d_prm, c->d_xs0, c->d_rec, C, sc, nnw,
+                              scw ? c->d_pre : nullptr, c->d_q, c->d_qctl, c->d_status, stats),
                 "main launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
@@ -141,7 +147,7 @@ int graph_for(Ctx *c, const Shape &sh, int C, hipGraphExec_t *out) {
 int run_batched(Ctx *c, int64_t nsteps) {
   const Shape sh = batched_shape(c);
   if (!(sh.P == 1 || sh.P == 2 || sh.P == 4)) return c->fail(MINT_EINVAL, "tps_per_lane must be 1, 2 or 4");
-  if (sh.block < 64 || sh.block > 1024 || (sh.block & 63)) return c->fail(MINT_EINVAL, "block_size must be a multiple of 64 in [64,1024]");
+  if (sh.block < 64 || sh.block > 512 || (sh.block & 63)) return c->fail(MINT_EINVAL, "block_size must be a multiple of 64 in [64,512]");
   if (sh.C < 1 || sh.C > 4096) return c->fail(MINT_EINVAL, "chunk_steps must be in [1,4096]");
   int rc = c->ensure_rec(sh.C);
   if (rc) return rc;
@@ -303,7 +309,7 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
     for (int d = 0; d < 6; ++d) hipMemcpy(c->d_tp[d], stp[d], tb, hipMemcpyDeviceToDevice);
   };
   const mint_config saved = c->cfg;
-  const int blocks[] = {64, 128, 256, 512, 1024};
+  const int blocks[] = {64, 128, 256, 512};
   const int Ps[] = {1, 2, 4};
   const int Cs[] = {32, 64, 128};
   const int probe = 128;
