@@ -53,6 +53,8 @@ void Ctx::free_device() {
   }
   graphs.clear();
   d_prm = d_mx = d_mv = d_xs0 = d_rec = nullptr;
+  d_xs0b = d_recb = nullptr;
+  rec2_steps_alloc = 0;
   for (auto &p : d_tp) p = nullptr;
   d_tp_sys = nullptr;
   d_pre = nullptr;
@@ -90,6 +92,28 @@ int Ctx::ensure_rec(int C) {
   int rc = dalloc((void **)&d_rec, bytes, "ephemeris");
   if (rc) return rc;
   rec_steps_alloc = C;
+  return MINT_OK;
+}
+
+// second ephemeris buffer (and chunk-start positions) for the pipeline
+int Ctx::ensure_rec2(int C) {
+  if (C <= rec2_steps_alloc) return MINT_OK;
+  for (double **p : {&d_recb, &d_xs0b}) {
+    if (*p) {
+      hipFree(*p);
+      allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)*p));
+      *p = nullptr;
+    }
+  }
+  for (auto &g : graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  graphs.clear();
+  const size_t bytes = (size_t)S * (size_t)C * (size_t)rec_size(NM) * sizeof(double);
+  int rc = dalloc((void **)&d_recb, bytes, "ephemeris (second buffer)");
+  if (rc) return rc;
+  if ((rc = dalloc((void **)&d_xs0b, (size_t)S * NM * 3 * sizeof(double), "chunk start positions (second buffer)")))
+    return rc;
+  rec2_steps_alloc = C;
   return MINT_OK;
 }
 
@@ -236,7 +260,8 @@ int mint_create(const mint_config *cfg, void **out) {
     delete c;
     return MINT_EINVAL;
   }
-  if (hipSetDevice(k.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipSetDevice(k.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipStreamCreate(&c->stream2) != hipSuccess) {
     g_create_error = "mint_create: cannot initialise device";
     delete c;
     return MINT_EHIP;
@@ -253,9 +278,13 @@ void mint_destroy(void *ctx) {
   hipSetDevice(c->cfg.device);
   hipStreamSynchronize(c->stream);
   c->free_device();
+  hipStreamSynchronize(c->stream2);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->ev_eph) hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_tp) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
+  hipStreamDestroy(c->stream2);
   delete c;
 }
 

@@ -13,7 +13,8 @@ namespace mint {
 // A captured launch window (ephemeris + queue reset + main + encounter) for
 // one window length C; replayed once per window.
 struct GraphEntry {
-  int C = 0;
+  long long nsteps = 0;
+  int block = 0, P = 0, C = 0;  // launch shape the graph was captured with
   hipGraphExec_t exec = nullptr;
 };
 
@@ -21,7 +22,9 @@ struct Ctx {
   mint_config cfg;
   std::string err;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // ephemeris runs one window ahead on this stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> ev_eph, ev_tp;  // per-window completion events of the pipeline
 
   // batch shape
   int S = 0, NM = 0;
@@ -34,6 +37,7 @@ struct Ctx {
   // device buffers
   std::vector<void *> allocs;
   double *d_prm = nullptr, *d_mx = nullptr, *d_mv = nullptr, *d_xs0 = nullptr, *d_rec = nullptr;
+  double *d_xs0b = nullptr, *d_recb = nullptr;  // second buffer of the pipelined batched form
   double *d_tp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int *d_tp_sys = nullptr;
   unsigned char *d_pre = nullptr;  // per-TP pre-bucket flag of the current window
@@ -46,6 +50,7 @@ struct Ctx {
   unsigned long long *d_stats = nullptr;
   float *d_nnw_kep = nullptr, *d_nnw_sc = nullptr;
   int rec_steps_alloc = 0;
+  int rec2_steps_alloc = 0;
 
   // learned parts (host copies; empty = not trained / not set)
   std::vector<float> w_kep, w_sc;
@@ -62,6 +67,7 @@ struct Ctx {
   void free_device();
   int dalloc(void **p, size_t bytes, const char *what);
   int ensure_rec(int C);
+  int ensure_rec2(int C);
   int ensure_tiles(int block, int P);
   StepConst step_const() const;
   TPArrays tp_arrays() const;

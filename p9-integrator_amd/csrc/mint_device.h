@@ -381,7 +381,7 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
 // Replaces modified_midpoint (src/sim.cuh:445-500), whose half-updated
 // velocity does not have an even error expansion.
 template <int NM>
-__device__ __noinline__ void gragg_tp(const double *y0, const double *f0, double t0, double H,
+__device__ __forceinline__ void gragg_tp(const double *y0, const double *f0, double t0, double H,
                                       int N, const double *__restrict__ Y,
                                       const double *__restrict__ W,
                                       const double *__restrict__ P4, double h,
@@ -419,12 +419,15 @@ __device__ __forceinline__ double bs_fac(int j) {
 // Aitken-Neville extrapolation in H^2, error |T_kk - T_k,k-1| scaled by |x0|,
 // |v0| against tol; on failure the interval is halved.  Replaces the fixed
 // 3-row richardson_extrapolation of src/sim.cuh:502-551.  Returns status bits.
+// The extrapolation tableau element (row, i) lives at Rt[(row * 6 + i) * rs]:
+// a private array (rs = 1) or a lane's column of an LDS block (rs = lanes).
 template <int NM>
-__device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
-                                             const double *__restrict__ W,
-                                             const double *__restrict__ P4,
-                                             const double *__restrict__ prm, double mu,
-                                             double eps2, double tol, int &nrhs) {
+__device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
+                                                const double *__restrict__ W,
+                                                const double *__restrict__ P4,
+                                                const double *__restrict__ prm, double mu,
+                                                double eps2, double tol, int &nrhs, double *Rt,
+                                                int rs) {
   uint32_t st = 0;
   double t = 0.0, H = h;
   int depth = 0;
@@ -435,24 +438,27 @@ __device__ __noinline__ uint32_t bs_drift_tp(double *yv, double h, const double 
     nrhs += 1;
     const double xs = sqrt(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
     const double vs = sqrt(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
-    double R[KMAX][6];
     double cur[6];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
       gragg_tp<NM>(yv, f0, t, H, 2 << k, Y, W, P4, h, prm, mu, eps2, cur, nrhs);
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
+#pragma unroll
         for (int i = 0; i < 6; ++i) {
-          const double prev = R[j - 1][i];
-          R[j - 1][i] = cur[i];
+          double *slot = Rt + ((j - 1) * 6 + i) * rs;
+          const double prev = *slot;
+          *slot = cur[i];
           cur[i] = cur[i] + (cur[i] - prev) / fac;
         }
       }
-      for (int i = 0; i < 6; ++i) R[k][i] = cur[i];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) Rt[(k * 6 + i) * rs] = cur[i];
       if (k >= 1) {
         double err = 0.0;
+#pragma unroll
         for (int i = 0; i < 6; ++i) {
-          const double e = fabs(cur[i] - R[k - 1][i]) / (i < 3 ? xs : vs);
+          const double e = fabs(cur[i] - Rt[((k - 1) * 6 + i) * rs]) / (i < 3 ? xs : vs);
           err = e > err ? e : err;
         }
         if (err <= tol) {

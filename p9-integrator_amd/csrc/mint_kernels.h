@@ -271,8 +271,10 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
     z = fma(sc.hh, r[R_Q1(NM) + 2], z);
     if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
       double yv[6] = {x, y, z, vx, vy, vz};
+      double Rt[KMAX * 6];
       int nr = 0;
-      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
+                            Rt, 1);
       x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
       n_bs = 1;
       n_rhs = (unsigned)nr;
@@ -471,23 +473,32 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// BS queue consumer.  Lanes take entries with an atomic head counter so long
-// and short jobs balance across the grid; each entry is carried from its step
-// k0 to the end of the window, with the exact check deciding Kepler or BS at
-// every step.
+// BS queue consumer.  The queue is short (tens to thousands of entries) and
+// each entry is a sequential chain of up to C steps, so entries are spread
+// over every wave of the grid: wave w takes entries w*L .. w*L+L-1 with
+// L = ceil(count / waves), one per lane, instead of packing them into a few
+// full waves.  Each entry is carried from its step k0 to the end of the
+// window, with the exact check deciding Kepler or BS at every step.
 template <int NM, int GUESS>
-__global__ void __launch_bounds__(256) tp_encounter_kernel(
+__global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
     unsigned int *__restrict__ qhead, TPArrays tp, const double *__restrict__ prm_all,
     const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
     const float *__restrict__ nnw, unsigned int *__restrict__ status,
     unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
+  // BS extrapolation tableau of each lane, lane-interleaved in LDS (one-wave blocks)
+  __shared__ double Rsh[KMAX * 6 * 64];
+  double *Rt = Rsh + (threadIdx.x & 63u);
   const unsigned int n = *qcount;
   unsigned int n_kep = 0, n_newton = 0, n_bs = 0, n_rhs = 0, n_step = 0;
-  for (;;) {
-    const unsigned int e = atomicAdd(qhead, 1u);
-    if (e >= n) break;
+  const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
+  const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned int lane = threadIdx.x & 63u;
+  unsigned int L = (n + nw - 1u) / nw;
+  L = L < 1u ? 1u : (L > 64u ? 64u : L);
+  (void)qhead;
+  for (unsigned int e = w * L + lane; lane < L && e < n; e += nw * L) {
     const QEntry en = q[e];
     const int s = en.sys;
     const double *prm = prm_all + (size_t)s * prm_size(NM);
@@ -514,7 +525,8 @@ __global__ void __launch_bounds__(256) tp_encounter_kernel(
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
-        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr);
+        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
+                              Rt, 64);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
         ++n_bs;
         n_rhs += (unsigned)nr;

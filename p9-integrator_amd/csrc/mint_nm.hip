@@ -81,7 +81,7 @@ hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const 
                           const double *prm, const double *xs0, const double *rec, int C,
                           StepConst sc, const float *nnw, unsigned int *status,
                           unsigned long long *stats) {
-  if (block <= 0 || block > 256 || (block & 63)) return hipErrorInvalidValue;
+  if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
   switch (guess) {
     case 0:
       tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,

@@ -109,9 +109,17 @@ __global__ void encounter_trace_kernel(TPArrays tp, const int *__restrict__ tp_s
 }
 
 // ---------------------------------------------------------------------------
-// Training kernels.  One thread per sample: forward pass with stored
-// activations, backward pass, gradient accumulated per block in LDS and then
-// added to the global gradient with one atomic per parameter per block.
+// Training kernels.  One thread per sample: forward pass with the hidden
+// activations kept in registers, then the gradient parameter by parameter:
+// each component is summed over the wave with shuffles and added once per
+// wave to an LDS accumulator, which is added to the global gradient once per
+// block.  Lanes past the end of the batch contribute zero.
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 template <int WHICH>
 struct Net;
@@ -120,12 +128,11 @@ template <>
 struct Net<0> {
   static constexpr int NP = KM_NP, IN = KM_IN;
   // out = W3 . tanh(W2 tanh(W1 in + b1) + b2) + b3 ; loss 0.5 (out - t)^2
-  __device__ static float fwd_bwd(const float *__restrict__ w, const float *in, float t, float *g) {
-    float a1[KM_H], h1[KM_H], h2[KM_H];
+  __device__ static float grad(const float *__restrict__ w, const float *in, float t, bool live, float *acc) {
+    float h1[KM_H], h2[KM_H], d2[KM_H], d1[KM_H];
     for (int i = 0; i < KM_H; ++i) {
       float a = w[KM_B1 + i];
       for (int k = 0; k < KM_IN; ++k) a = fmaf(w[KM_W1 + i * KM_IN + k], in[k], a);
-      a1[i] = a;
       h1[i] = tanhf(a);
     }
     float out = w[KM_B3];
@@ -135,26 +142,27 @@ struct Net<0> {
       h2[i] = tanhf(a);
       out = fmaf(w[KM_W3 + i], h2[i], out);
     }
-    (void)a1;
-    const float e = out - t;
-    if (g) {
-      float dh1[KM_H];
-      for (int k = 0; k < KM_H; ++k) dh1[k] = 0.f;
-      atomicAdd(&g[KM_B3], e);
-      for (int i = 0; i < KM_H; ++i) {
-        atomicAdd(&g[KM_W3 + i], e * h2[i]);
-        const float d2 = e * w[KM_W3 + i] * (1.f - h2[i] * h2[i]);
-        atomicAdd(&g[KM_B2 + i], d2);
-        for (int k = 0; k < KM_H; ++k) {
-          atomicAdd(&g[KM_W2 + i * KM_H + k], d2 * h1[k]);
-          dh1[k] = fmaf(d2, w[KM_W2 + i * KM_H + k], dh1[k]);
-        }
-      }
-      for (int k = 0; k < KM_H; ++k) {
-        const float d1 = dh1[k] * (1.f - h1[k] * h1[k]);
-        atomicAdd(&g[KM_B1 + k], d1);
-        for (int q = 0; q < KM_IN; ++q) atomicAdd(&g[KM_W1 + k * KM_IN + q], d1 * in[q]);
-      }
+    const float e = live ? out - t : 0.f;
+    for (int i = 0; i < KM_H; ++i) d2[i] = e * w[KM_W3 + i] * (1.f - h2[i] * h2[i]);
+    for (int k = 0; k < KM_H; ++k) {
+      float a = 0.f;
+      for (int i = 0; i < KM_H; ++i) a = fmaf(d2[i], w[KM_W2 + i * KM_H + k], a);
+      d1[k] = a * (1.f - h1[k] * h1[k]);
+    }
+    const bool l0 = (threadIdx.x & 63) == 0;
+    auto put = [&](int p, float g) {
+      g = wave_sum(g);
+      if (l0) atomicAdd(&acc[p], g);
+    };
+    put(KM_B3, e);
+    for (int i = 0; i < KM_H; ++i) put(KM_W3 + i, e * h2[i]);
+    for (int i = 0; i < KM_H; ++i) {
+      put(KM_B2 + i, d2[i]);
+      for (int k = 0; k < KM_H; ++k) put(KM_W2 + i * KM_H + k, d2[i] * h1[k]);
+    }
+    for (int k = 0; k < KM_H; ++k) {
+      put(KM_B1 + k, d1[k]);
+      for (int q = 0; q < KM_IN; ++q) put(KM_W1 + k * KM_IN + q, d1[k] * in[q]);
     }
     return 0.5f * e * e;
   }
@@ -165,7 +173,7 @@ template <>
 struct Net<1> {
   static constexpr int NP = SC_NP, IN = SC_IN;
   // logit = W2 . tanh(W1 in + b1) + b2 ; BCE with logits
-  __device__ static float fwd_bwd(const float *__restrict__ w, const float *in, float t, float *g) {
+  __device__ static float grad(const float *__restrict__ w, const float *in, float t, bool live, float *acc) {
     float h[SC_H];
     float out = w[SC_B2];
     for (int i = 0; i < SC_H; ++i) {
@@ -175,16 +183,19 @@ struct Net<1> {
       out = fmaf(w[SC_W2 + i], h[i], out);
     }
     const float p = 1.f / (1.f + expf(-out));
-    const float loss = fmaxf(out, 0.f) - out * t + log1pf(expf(-fabsf(out)));
-    if (g) {
-      const float e = p - t;
-      atomicAdd(&g[SC_B2], e);
-      for (int i = 0; i < SC_H; ++i) {
-        atomicAdd(&g[SC_W2 + i], e * h[i]);
-        const float d = e * w[SC_W2 + i] * (1.f - h[i] * h[i]);
-        atomicAdd(&g[SC_B1 + i], d);
-        for (int k = 0; k < SC_IN; ++k) atomicAdd(&g[SC_W1 + i * SC_IN + k], d * in[k]);
-      }
+    const float loss = live ? fmaxf(out, 0.f) - out * t + log1pf(expf(-fabsf(out))) : 0.f;
+    const float e = live ? p - t : 0.f;
+    const bool l0 = (threadIdx.x & 63) == 0;
+    auto put = [&](int q, float g) {
+      g = wave_sum(g);
+      if (l0) atomicAdd(&acc[q], g);
+    };
+    put(SC_B2, e);
+    for (int i = 0; i < SC_H; ++i) {
+      put(SC_W2 + i, e * h[i]);
+      const float d = e * w[SC_W2 + i] * (1.f - h[i] * h[i]);
+      put(SC_B1 + i, d);
+      for (int k = 0; k < SC_IN; ++k) put(SC_W1 + i * SC_IN + k, d * in[k]);
     }
     return loss;
   }
@@ -205,13 +216,14 @@ __global__ void __launch_bounds__(256) nn_grad_kernel(const float *__restrict__ 
   if (threadIdx.x == 0) lsum = 0.f;
   __syncthreads();
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) {
-    const long long s = perm ? perm[j] : j;
-    float in[N::IN];
-    for (int q = 0; q < N::IN; ++q) in[q] = feat[(size_t)s * N::IN + q];
-    const float l = N::fwd_bwd(w, in, target[s], g);
-    atomicAdd(&lsum, l);
-  }
+  const bool live = j < n;
+  float in[N::IN];
+  float t = 0.f;
+  const long long s = live ? (perm ? perm[j] : j) : 0;
+  for (int q = 0; q < N::IN; ++q) in[q] = live ? feat[(size_t)s * N::IN + q] : 0.f;
+  if (live) t = target[s];
+  const float l = wave_sum(N::grad(w, in, t, live, g));
+  if ((threadIdx.x & 63) == 0) atomicAdd(&lsum, l);
   __syncthreads();
   for (int i = threadIdx.x; i < N::NP; i += blockDim.x) atomicAdd(&grad[i], g[i]);
   if (threadIdx.x == 0) atomicAdd(loss_sum, lsum);

@@ -37,8 +37,9 @@ Shape batched_shape(const Ctx *c) {
   return s;
 }
 
+// encounter kernel: one-wave blocks, enough to put a wave on every SIMD twice
 int enc_grid(const Ctx *c) {
-  long long g = (c->Ntp + 255) / 256;
+  long long g = (c->Ntp + 63) / 64;
   return (int)std::max(1LL, std::min(g, 2048LL));
 }
 
@@ -73,8 +74,21 @@ struct KTimer {
   }
 };
 
-// One launch window of C steps of the batched form.
-int launch_window(Ctx *c, const Shape &sh, int C, KTimer *kt) {
+// Ephemeris of one launch window into buffer (xs0, rec) on stream st.
+int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, KTimer *kt) {
+  const StepConst sc = c->step_const();
+  if (kt) kt->start();
+  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, c->d_status, sc,
+                                  c->cfg.central_mass),
+                  "ephemeris launch");
+  if (rc) return rc;
+  if (kt) kt->stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
+  return MINT_OK;
+}
+
+// TP side of one launch window of C steps (queue reset, optional learned
+// pre-bucketing, main kernel, encounter queue) on the context stream.
+int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *rec, KTimer *kt) {
   const StepConst sc = c->step_const();
   const TPArrays tp = c->tp_arrays();
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
@@ -83,32 +97,25 @@ int launch_window(Ctx *c, const Shape &sh, int C, KTimer *kt) {
   const int tpx = (c->ntiles + 7) / 8;
   const int nblocks = 8 * tpx;
   int rc;
-  if (kt) kt->start();
-  rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, c->d_xs0, c->d_rec,
-                              c->d_status, sc, c->cfg.central_mass),
-              "ephemeris launch");
-  if (rc) return rc;
-  if (kt) kt->stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
   zero_u32_kernel<<<1, 64, 0, c->stream>>>(c->d_qctl, 4);
   if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
   if (scw && c->Ntp > 0) {
-    rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, c->d_xs0, c->d_rec, C, sc,
-                                   scw, c->score_thresh, c->d_pre, c->d_q, c->d_qctl, stats),
+    rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs0, rec, C, sc, scw,
+                                   c->score_thresh, c->d_pre, c->d_q, c->d_qctl, stats),
                 "prebucket launch");
     if (rc) return rc;
   }
   if (c->ntiles > 0) {
     if (kt) kt->start();
-    rc = c->hip(c->L->tp_main(c->stream, c->cfg.kepler_guess, sh.P, sh.block, nblocks, c->d_tiles,
-                              c->ntiles, tpx, tp, c->d_prm, c->d_xs0, c->d_rec, C, sc, nnw,
-                              scw ? c->d_pre : nullptr, c->d_q, c->d_qctl, c->d_status, stats),
+    rc = c->hip(c->L->tp_main(c->stream, c->cfg.kepler_guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx,
+                              tp, c->d_prm, xs0, rec, C, sc, nnw, scw ? c->d_pre : nullptr, c->d_q, c->d_qctl,
+                              c->d_status, stats),
                 "main launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
     if (kt) kt->start();
-    rc = c->hip(c->L->tp_encounter(c->stream, c->cfg.kepler_guess, enc_grid(c), 256, c->d_q,
-                                   c->d_qctl, c->d_qctl + 1, tp, c->d_prm, c->d_xs0, c->d_rec, C,
-                                   sc, nnw, c->d_status, stats),
+    rc = c->hip(c->L->tp_encounter(c->stream, c->cfg.kepler_guess, enc_grid(c), 64, c->d_q, c->d_qctl,
+                                   c->d_qctl + 1, tp, c->d_prm, xs0, rec, C, sc, nnw, c->d_status, stats),
                 "encounter launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
@@ -116,15 +123,63 @@ int launch_window(Ctx *c, const Shape &sh, int C, KTimer *kt) {
   return MINT_OK;
 }
 
-int graph_for(Ctx *c, const Shape &sh, int C, hipGraphExec_t *out) {
+hipEvent_t window_event(std::vector<hipEvent_t> &v, size_t i) {
+  while (v.size() <= i) {
+    hipEvent_t e = nullptr;
+    hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    v.push_back(e);
+  }
+  return v[i];
+}
+
+// Enqueue nsteps of the batched form.  Pipelined: the ephemeris of window
+// w+1 runs on the second stream while the TP kernels of window w run on the
+// context stream, with the two ephemeris buffers alternating; window w's TP
+// kernels wait for its ephemeris, and the ephemeris of window w+2 waits for
+// window w's TP kernels (it overwrites their buffer).
+int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTimer *kt) {
+  std::vector<int> win;
+  for (int64_t done = 0; done < nsteps;) {
+    const int C = (int)std::min<int64_t>(sh.C, nsteps - done);
+    win.push_back(C);
+    done += C;
+  }
+  double *xs[2] = {c->d_xs0, c->d_xs0b};
+  double *rec[2] = {c->d_rec, c->d_recb};
+  int rc;
+  if (!pipelined || win.size() < 2) {
+    for (int C : win) {
+      if ((rc = window_eph(c, c->stream, C, xs[0], rec[0], kt))) return rc;
+      if ((rc = window_tp(c, sh, C, xs[0], rec[0], kt))) return rc;
+    }
+    return MINT_OK;
+  }
+  hipEvent_t fork = window_event(c->ev_eph, 0);
+  if ((rc = c->hip(hipEventRecord(fork, c->stream), "fork event"))) return rc;
+  if ((rc = c->hip(hipStreamWaitEvent(c->stream2, fork, 0), "fork wait"))) return rc;
+  for (size_t w = 0; w < win.size(); ++w) {
+    const int b = (int)(w & 1);
+    if (w >= 2 && (rc = c->hip(hipStreamWaitEvent(c->stream2, window_event(c->ev_tp, w - 2), 0), "buffer wait")))
+      return rc;
+    if ((rc = window_eph(c, c->stream2, win[w], xs[b], rec[b], nullptr))) return rc;
+    hipEvent_t e = window_event(c->ev_eph, w + 1);
+    if ((rc = c->hip(hipEventRecord(e, c->stream2), "ephemeris event"))) return rc;
+    if ((rc = c->hip(hipStreamWaitEvent(c->stream, e, 0), "ephemeris wait"))) return rc;
+    if ((rc = window_tp(c, sh, win[w], xs[b], rec[b], nullptr))) return rc;
+    if ((rc = c->hip(hipEventRecord(window_event(c->ev_tp, w), c->stream), "window event"))) return rc;
+  }
+  return MINT_OK;
+}
+
+int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out) {
   for (auto &g : c->graphs)
-    if (g.C == C) {
+    if (g.nsteps == nsteps && g.block == sh.block && g.P == sh.P && g.C == sh.C) {
       *out = g.exec;
       return MINT_OK;
     }
   int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
-  rc = launch_window(c, sh, C, nullptr);
+  rc = enqueue_batched(c, sh, nsteps, true, nullptr);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
@@ -136,8 +191,15 @@ int graph_for(Ctx *c, const Shape &sh, int C, hipGraphExec_t *out) {
   e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if ((rc = c->hip(e, "graph instantiate"))) return rc;
+  if (c->graphs.size() >= 8) {
+    hipGraphExecDestroy(c->graphs.front().exec);
+    c->graphs.erase(c->graphs.begin());
+  }
   GraphEntry ge;
-  ge.C = C;
+  ge.nsteps = nsteps;
+  ge.block = sh.block;
+  ge.P = sh.P;
+  ge.C = sh.C;
   ge.exec = ex;
   c->graphs.push_back(ge);
   *out = ex;
@@ -151,22 +213,16 @@ int run_batched(Ctx *c, int64_t nsteps) {
   if (sh.C < 1 || sh.C > 4096) return c->fail(MINT_EINVAL, "chunk_steps must be in [1,4096]");
   int rc = c->ensure_rec(sh.C);
   if (rc) return rc;
+  if ((rc = c->ensure_rec2(sh.C))) return rc;
   if ((rc = c->ensure_tiles(sh.block, sh.P))) return rc;
-  const bool use_graph = c->cfg.use_graph && !timing_on(c);
-  KTimer kt(c);
-  int64_t done = 0;
-  while (done < nsteps) {
-    const int C = (int)std::min<int64_t>(sh.C, nsteps - done);
-    if (use_graph) {
-      hipGraphExec_t ex;
-      if ((rc = graph_for(c, sh, C, &ex))) return rc;
-      if ((rc = c->hip(hipGraphLaunch(ex, c->stream), "graph launch"))) return rc;
-    } else {
-      if ((rc = launch_window(c, sh, C, timing_on(c) ? &kt : nullptr))) return rc;
-    }
-    done += C;
+  if (timing_on(c)) {
+    KTimer kt(c);
+    return enqueue_batched(c, sh, nsteps, false, &kt);
   }
-  return MINT_OK;
+  if (!c->cfg.use_graph) return enqueue_batched(c, sh, nsteps, true, nullptr);
+  hipGraphExec_t ex;
+  if ((rc = graph_for(c, sh, nsteps, &ex))) return rc;
+  return c->hip(hipGraphLaunch(ex, c->stream), "graph launch");
 }
 
 // Plain (groups of systems_per_launch systems) and flat (all systems): one
