@@ -186,8 +186,11 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best);
 
 /* Learned parts (FP32).  Weights live in the context.
  * Kepler warm-start MLP: 4 -> 32 -> 32 -> 1, tanh.  Scorer: 6 -> 16 -> 1, tanh/logistic.
- * mint_trace_*: run nsteps with trace logging into a device ring buffer of
- * capacity `cap` samples, then copy samples out (features then target). */
+ * mint_trace_*: run nsteps while logging samples into a device buffer of
+ * capacity `cap`, then copy them out (features, then target or label).
+ * Kepler: (features, c) of sampled TP and massive-body drifts.  Encounter:
+ * per launch window, every (TP, body) pair with label 1 (the exact check fired
+ * in the window) and a hashed one-in-64 of the pairs with label 0. */
 int mint_trace_kepler(void *ctx, int64_t nsteps, int64_t cap, float *feat /*cap*4*/,
                       float *target /*cap*/, int64_t *n_out);
 int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat /*cap*6*/,

@@ -54,6 +54,8 @@ void Ctx::free_device() {
   graphs.clear();
   d_prm = d_mx = d_mv = d_xs0 = d_rec = nullptr;
   d_xs0b = d_recb = nullptr;
+  d_pc = nullptr;
+  d_need = nullptr;
   rec2_steps_alloc = 0;
   for (auto &p : d_tp) p = nullptr;
   d_tp_sys = nullptr;
@@ -88,9 +90,15 @@ int Ctx::ensure_rec(int C) {
       if (g.exec) hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
+  if (d_pc) {
+    hipFree(d_pc);
+    allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)d_pc));
+    d_pc = nullptr;
+  }
   const size_t bytes = (size_t)S * (size_t)C * (size_t)rec_size(NM) * sizeof(double);
   int rc = dalloc((void **)&d_rec, bytes, "ephemeris");
   if (rc) return rc;
+  if ((rc = dalloc((void **)&d_pc, (size_t)S * C * pc_size(NM) * sizeof(double), "perturber cache"))) return rc;
   rec_steps_alloc = C;
   return MINT_OK;
 }
@@ -398,6 +406,7 @@ int mint_upload(void *ctx, const mint_state *st) {
     if ((rc = c->dalloc((void **)&c->d_tp[d], (size_t)N * sizeof(double), "TP state"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_tp_sys, (size_t)N * sizeof(int), "TP system index"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_pre, (size_t)N, "pre-bucket flags"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_need, (size_t)S * sizeof(unsigned int), "cache flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_q, (size_t)std::max<int64_t>(N, 1) * sizeof(QEntry), "BS queue"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_qctl, 4 * sizeof(unsigned int), "queue counters"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_status, (size_t)S * sizeof(unsigned int), "status"))) return rc;

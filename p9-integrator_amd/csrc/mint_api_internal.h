@@ -38,6 +38,8 @@ struct Ctx {
   std::vector<void *> allocs;
   double *d_prm = nullptr, *d_mx = nullptr, *d_mv = nullptr, *d_xs0 = nullptr, *d_rec = nullptr;
   double *d_xs0b = nullptr, *d_recb = nullptr;  // second buffer of the pipelined batched form
+  double *d_pc = nullptr;          // perturber cache of the current window
+  unsigned int *d_need = nullptr;  // per-system: a TP entered the encounter queue this window
   double *d_tp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int *d_tp_sys = nullptr;
   unsigned char *d_pre = nullptr;  // per-TP pre-bucket flag of the current window

@@ -37,6 +37,14 @@ __host__ __device__ constexpr int P_PRCLO(int nm) { return 4 * nm; }
 __host__ __device__ constexpr int P_PRCWINV(int nm) { return 4 * nm + nm * nm; }
 __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; }
 
+// Perturber cache: for every system and step, the massive positions
+// Kepler-drifted from (Y, W) by tau = m (h/16), m = 1..16, the substep times of
+// BS rows N = 2, 4, 8, 16 over the undivided step.  A BS right-hand side at one
+// of these times reads them instead of solving Kepler's equation (the same
+// function of the same inputs, so the same bits).  [s][k][m-1][3 NM] doubles.
+constexpr int PC_PTS = 16;
+__host__ __device__ constexpr int pc_size(int nm) { return PC_PTS * 3 * nm; }
+
 // Ephemeris record of one step (doubles): what every TP phase of that step
 // reads from the massive subsystem (the per-phase reads of block 0's shared
 // memory in src/sim.cuh:682-741).
@@ -45,18 +53,12 @@ __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; 
 //   [6, 6+3NM)       Y: massive positions after jump 1     (encounter test, BS)
 //   [6+3NM, 6+6NM)   W: massive velocities at drift start  (BS perturber drift)
 //   [6+6NM, 6+9NM)   Xe: massive positions after jump 2    (closing kick)
-//   [6+9NM, 6+21NM)  P4: perturber positions Kepler-drifted from (Y, W) by
-//                    tau = h/4, 2(h/4), 3(h/4) and h, the substep times of the
-//                    first two BS rows; a BS right-hand side at one of these
-//                    times reads them instead of solving Kepler's equation
-//                    (same function of the same inputs, so the same bits)
-__host__ __device__ constexpr int rec_size(int nm) { return ((6 + 21 * nm) + 1) & ~1; }
+__host__ __device__ constexpr int rec_size(int nm) { return ((6 + 9 * nm) + 1) & ~1; }
 __host__ __device__ constexpr int R_Q1(int) { return 0; }
 __host__ __device__ constexpr int R_Q2(int) { return 3; }
 __host__ __device__ constexpr int R_Y(int) { return 6; }
 __host__ __device__ constexpr int R_W(int nm) { return 6 + 3 * nm; }
 __host__ __device__ constexpr int R_XE(int nm) { return 6 + 6 * nm; }
-__host__ __device__ constexpr int R_P4(int nm) { return 6 + 9 * nm; }
 
 struct StepConst {
   double h;     // step
@@ -341,23 +343,16 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
   const double r = sqrt(r2);
   const double w0 = mu / (r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
-  const double hs4 = h / 4.0;
-  int ci = -1;
-  if (tau == hs4)
-    ci = 0;
-  else if (tau == 2.0 * hs4)
-    ci = 1;
-  else if (tau == 3.0 * hs4)
-    ci = 2;
-  else if (tau == h)
-    ci = 3;
+  const double hs16 = h / (double)PC_PTS;
+  int ci = (int)(tau * ((double)PC_PTS / h) + 0.5);
+  if (!(ci >= 1 && ci <= PC_PTS && (double)ci * hs16 == tau)) ci = 0;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     double px = Y[3 * j + 0], py = Y[3 * j + 1], pz = Y[3 * j + 2];
-    if (ci >= 0) {
-      px = P4[ci * 3 * NM + 3 * j + 0];
-      py = P4[ci * 3 * NM + 3 * j + 1];
-      pz = P4[ci * 3 * NM + 3 * j + 2];
+    if (ci > 0 && P4 != nullptr) {
+      px = P4[(ci - 1) * 3 * NM + 3 * j + 0];
+      py = P4[(ci - 1) * 3 * NM + 3 * j + 1];
+      pz = P4[(ci - 1) * 3 * NM + 3 * j + 2];
     } else if (tau != 0.0) {
       double qx = W[3 * j + 0], qy = W[3 * j + 1], qz = W[3 * j + 2];
       bool ok;

@@ -159,34 +159,8 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
       r[R_Y(NM) + i] = X[i];
       r[R_W(NM) + i] = V[i];
     }
-    // perturber positions at the substep times of the first BS rows
-    {
-      const double hs4 = sc.h / 4.0;
-      for (int q = 1; q <= 3; ++q) {
-        const double tau = 0.0 + (double)q * hs4;
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-          double px = X[3 * j], py = X[3 * j + 1], pz = X[3 * j + 2];
-          double qx = V[3 * j], qy = V[3 * j + 1], qz = V[3 * j + 2];
-          bool ok;
-          kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, tau, ok);
-          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 0] = px;
-          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 1] = py;
-          r[R_P4(NM) + (q - 1) * 3 * NM + 3 * j + 2] = pz;
-        }
-      }
-    }
     if (encounter_massive<NM>(X, prm)) {
       st |= ST_MASSIVE_ENC;
-      for (int j = 0; j < NM; ++j) {
-        double px = X[3 * j], py = X[3 * j + 1], pz = X[3 * j + 2];
-        double qx = V[3 * j], qy = V[3 * j + 1], qz = V[3 * j + 2];
-        bool ok;
-        kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, sc.h, ok);
-        r[R_P4(NM) + 9 * NM + 3 * j + 0] = px;
-        r[R_P4(NM) + 9 * NM + 3 * j + 1] = py;
-        r[R_P4(NM) + 9 * NM + 3 * j + 2] = pz;
-      }
       double sv[6 * NM];
       for (int j = 0; j < NM; ++j)
         for (int c = 0; c < 3; ++c) {
@@ -206,9 +180,6 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
         kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
                             V[3 * j + 2], sc.mu, sc.h, ok);
         if (!ok) st |= ST_NEWTON;
-        r[R_P4(NM) + 9 * NM + 3 * j + 0] = X[3 * j];
-        r[R_P4(NM) + 9 * NM + 3 * j + 1] = X[3 * j + 1];
-        r[R_P4(NM) + 9 * NM + 3 * j + 2] = X[3 * j + 2];
       }
     }
     momentum_q<NM>(V, prm, M, Q);
@@ -235,6 +206,39 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
   }
   if (!fin) st |= ST_NAN;
   if (st) atomicOr(&status[s], st);
+}
+
+// ---------------------------------------------------------------------------
+// Perturber cache: one lane per (system, step, substep point, body), each a
+// Kepler drift of the body's drift-start state (Y, W) from the ephemeris.
+// Only systems flagged in need[] (a TP of theirs entered the encounter queue
+// in this window) are computed.
+template <int NM>
+__global__ void __launch_bounds__(256) pcache_kernel(int s_begin, int s_end, int C,
+                                                     const double *__restrict__ rec,
+                                                     const unsigned int *__restrict__ need,
+                                                     double *__restrict__ pc, StepConst sc) {
+  constexpr int REC = rec_size(NM);
+  const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NM;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int j = (int)(t % NM);
+  long long u = t / NM;
+  const int m = (int)(u % PC_PTS) + 1;
+  u /= PC_PTS;
+  const int k = (int)(u % C);
+  const int s = s_begin + (int)(u / C);
+  if (need[s] == 0u) return;
+  const double *r = rec + ((size_t)s * C + k) * REC;
+  double px = r[R_Y(NM) + 3 * j], py = r[R_Y(NM) + 3 * j + 1], pz = r[R_Y(NM) + 3 * j + 2];
+  double qx = r[R_W(NM) + 3 * j], qy = r[R_W(NM) + 3 * j + 1], qz = r[R_W(NM) + 3 * j + 2];
+  const double tau = (double)m * (sc.h / (double)PC_PTS);
+  bool ok;
+  kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, tau, ok);
+  double *o = pc + (((size_t)s * C + k) * PC_PTS + (m - 1)) * 3 * NM + 3 * j;
+  o[0] = px;
+  o[1] = py;
+  o[2] = pz;
 }
 
 // ---------------------------------------------------------------------------
@@ -273,7 +277,7 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
       double yv[6] = {x, y, z, vx, vy, vz};
       double Rt[KMAX * 6];
       int nr = 0;
-      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
+      st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), nullptr, prm, sc.mu, sc.eps2, sc.tol, nr,
                             Rt, 1);
       x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
       n_bs = 1;
@@ -337,8 +341,8 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
     const double *__restrict__ prm_all, const double *__restrict__ xs0,
     const double *__restrict__ rec, int C, StepConst sc, const float *__restrict__ nnw,
     const unsigned char *__restrict__ pre, QEntry *__restrict__ q,
-    unsigned int *__restrict__ qcount, unsigned int *__restrict__ status,
-    unsigned long long *__restrict__ stats) {
+    unsigned int *__restrict__ qcount, unsigned int *__restrict__ need,
+    unsigned int *__restrict__ status, unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so the tiles of one system share an L2.
@@ -394,6 +398,7 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
       if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
         const unsigned int slot = atomicAdd(qcount, 1u);
         q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
+        need[s] = 1u;
         act[p] = false;
         ++n_defer;
         n_idle += (unsigned)(C - k);
@@ -441,7 +446,7 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
     long long n, const int *__restrict__ tp_sys, TPArrays tp, const double *__restrict__ prm_all,
     const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
     const float *__restrict__ scw, float thresh, unsigned char *__restrict__ pre,
-    QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
+    QEntry *__restrict__ q, unsigned int *__restrict__ qcount, unsigned int *__restrict__ need,
     unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -466,6 +471,7 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
     if (go) {
       const unsigned int slot = atomicAdd(qcount, 1u);
       q_put(q + slot, (int)i, s, 0, 0, x, y, z, vx, vy, vz);
+      need[s] = 1u;
       n_pre = 1;
     }
   }
@@ -483,8 +489,8 @@ template <int NM, int GUESS>
 __global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
     unsigned int *__restrict__ qhead, TPArrays tp, const double *__restrict__ prm_all,
-    const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
-    const float *__restrict__ nnw, unsigned int *__restrict__ status,
+    const double *__restrict__ xs0, const double *__restrict__ rec, const double *__restrict__ pc,
+    int C, StepConst sc, const float *__restrict__ nnw, unsigned int *__restrict__ status,
     unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   // BS extrapolation tableau of each lane, lane-interleaved in LDS (one-wave blocks)
@@ -525,7 +531,8 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
-        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM), r + R_P4(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
+        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
+                              pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
                               Rt, 64);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
         ++n_bs;

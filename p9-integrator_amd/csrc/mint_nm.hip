@@ -52,10 +52,10 @@ hipError_t main_pg(hipStream_t st, int block, int nblocks, const Tile *tiles, in
                    int tiles_per_xcd, TPArrays tp, const double *prm, const double *xs0,
                    const double *rec, int C, StepConst sc, const float *nnw,
                    const unsigned char *pre, QEntry *q, unsigned int *qcount,
-                   unsigned int *status, unsigned long long *stats) {
+                   unsigned int *need, unsigned int *status, unsigned long long *stats) {
   tp_main_kernel<NMV, P, G><<<nblocks, block, 0, st>>>(tiles, ntiles, tiles_per_xcd, tp, prm, xs0,
-                                                       rec, C, sc, nnw, pre, q, qcount, status,
-                                                       stats);
+                                                       rec, C, sc, nnw, pre, q, qcount, need,
+                                                       status, stats);
   return hipGetLastError();
 }
 
@@ -63,12 +63,12 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
                      int ntiles, int tiles_per_xcd, TPArrays tp, const double *prm,
                      const double *xs0, const double *rec, int C, StepConst sc, const float *nnw,
                      const unsigned char *pre, QEntry *q, unsigned int *qcount,
-                     unsigned int *status, unsigned long long *stats) {
+                     unsigned int *need, unsigned int *status, unsigned long long *stats) {
   if (block <= 0 || block > 512 || (block & 63)) return hipErrorInvalidValue;
 #define MINT_MAIN(PP, GG)                                                                       \
   if (P == PP && guess == GG)                                                                   \
     return main_pg<PP, GG>(st, block, nblocks, tiles, ntiles, tiles_per_xcd, tp, prm, xs0, rec, \
-                           C, sc, nnw, pre, q, qcount, status, stats);
+                           C, sc, nnw, pre, q, qcount, need, status, stats);
   MINT_MAIN(1, 0) MINT_MAIN(1, 1) MINT_MAIN(1, 2)
   MINT_MAIN(2, 0) MINT_MAIN(2, 1) MINT_MAIN(2, 2)
   MINT_MAIN(4, 0) MINT_MAIN(4, 1) MINT_MAIN(4, 2)
@@ -78,22 +78,22 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
 
 hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
                           const unsigned int *qcount, unsigned int *qhead, TPArrays tp,
-                          const double *prm, const double *xs0, const double *rec, int C,
-                          StepConst sc, const float *nnw, unsigned int *status,
-                          unsigned long long *stats) {
+                          const double *prm, const double *xs0, const double *rec,
+                          const double *pc, int C, StepConst sc, const float *nnw,
+                          unsigned int *status, unsigned long long *stats) {
   if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
   switch (guess) {
     case 0:
-      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
-                                                          sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+                                                          C, sc, nnw, status, stats);
       break;
     case 1:
-      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
-                                                          sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+                                                          C, sc, nnw, status, stats);
       break;
     case 2:
-      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, C,
-                                                          sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+                                                          C, sc, nnw, status, stats);
       break;
     default:
       return hipErrorInvalidValue;
@@ -104,10 +104,19 @@ hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const 
 hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
                           const double *prm, const double *xs0, const double *rec, int C,
                           StepConst sc, const float *scw, float thresh, unsigned char *pre,
-                          QEntry *q, unsigned int *qcount, unsigned long long *stats) {
+                          QEntry *q, unsigned int *qcount, unsigned int *need,
+                          unsigned long long *stats) {
   if (n <= 0) return hipSuccess;
   tp_prebucket_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, pre, q, qcount, stats);
+      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, pre, q, qcount, need, stats);
+  return hipGetLastError();
+}
+
+hipError_t pcache_l(hipStream_t st, int s_begin, int s_end, int C, const double *rec,
+                    const unsigned int *need, double *pc, StepConst sc) {
+  const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NMV;
+  if (n <= 0) return hipSuccess;
+  pcache_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(s_begin, s_end, C, rec, need, pc, sc);
   return hipGetLastError();
 }
 
@@ -116,7 +125,8 @@ hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArra
 #define MINT_CAT2(a, b) a##b
 #define MINT_CAT(a, b) MINT_CAT2(a, b)
 const Launchers *MINT_CAT(launchers_nm, MINT_NM)() {
-  static const Launchers L = {ephemeris_l, tp_step_l, tp_main_l, tp_encounter_l, tp_prebucket_l};
+  static const Launchers L = {ephemeris_l, tp_step_l, tp_main_l, tp_encounter_l, tp_prebucket_l,
+                              pcache_l};
   return &L;
 }
 

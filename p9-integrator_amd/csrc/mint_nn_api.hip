@@ -49,16 +49,17 @@ __global__ void kepler_trace_kernel(const double *__restrict__ x, const double *
   target[slot] = (float)c;
 }
 
-// Encounter trace: for every `stride`-th TP, scorer features of each
-// (TP, body) pair at the window start and label = 1 if the exact check fires
-// for that pair at any step of the window along the TP's Kepler-drift path.
+// Encounter trace: for every TP, scorer features of each (TP, body) pair at
+// the window start and label = 1 if the exact check fires for that pair at any
+// step of the window along the TP's Kepler-drift path; all positive pairs and a
+// hashed one-in-64 of the negatives are written.
 template <int NM>
 __global__ void encounter_trace_kernel(TPArrays tp, const int *__restrict__ tp_sys, long long n,
                                        long long stride, const double *__restrict__ prm_all,
                                        const double *__restrict__ xs0, const double *__restrict__ rec,
                                        int C, StepConst sc, float *__restrict__ feat,
                                        float *__restrict__ label, unsigned int *__restrict__ counter,
-                                       unsigned int cap) {
+                                       unsigned int cap, unsigned int win) {
   constexpr int REC = rec_size(NM);
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long i = j * stride;
@@ -101,10 +102,17 @@ __global__ void encounter_trace_kernel(TPArrays tp, const int *__restrict__ tp_s
     vz = fma(sc.hh, az, vz);
   }
   for (int q = 0; q < NM; ++q) {
+    const bool pos = (hit >> q) & 1u;
+    // keep every positive pair and one negative in 64 (hash of TP, body, window)
+    unsigned int hsh = (unsigned int)i * 2654435761u ^ (unsigned int)(q + 1) * 40503u ^ win * 2246822519u;
+    hsh ^= hsh >> 15;
+    hsh *= 2246822519u;
+    hsh ^= hsh >> 13;
+    if (!pos && (hsh & 63u) != 0u) continue;
     const unsigned int slot = atomicAdd(counter, 1u);
     if (slot >= cap) return;
     for (int u = 0; u < SC_IN; ++u) feat[(size_t)slot * SC_IN + u] = f[q][u];
-    label[slot] = (hit >> q) & 1u ? 1.0f : 0.0f;
+    label[slot] = pos ? 1.0f : 0.0f;
   }
 }
 
@@ -424,8 +432,7 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
   int rc = c->ensure_rec(C);
   if (rc) return rc;
   const int nwin = (int)std::max<int64_t>(1, nsteps / C);
-  const long long per_win = std::max<long long>(1, cap / ((long long)nwin * c->NM));
-  const long long stride = std::max<long long>(1, c->Ntp / per_win);
+  const long long stride = 1;
   const StepConst sc = c->step_const();
   for (int w = 0; w < nwin; ++w) {
     // fill the ephemeris for the coming window without advancing: run the
@@ -444,11 +451,11 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
       if (c->NM == 5)
         encounter_trace_kernel<5><<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(
             c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_rec, C, sc,
-            (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap);
+            (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap, (unsigned)w);
       else
         encounter_trace_kernel<2><<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(
             c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_rec, C, sc,
-            (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap);
+            (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap, (unsigned)w);
       if ((rc = c->hip(hipGetLastError(), "encounter trace launch"))) return rc;
     }
     hipStreamSynchronize(c->stream);

@@ -98,10 +98,11 @@ int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *r
   const int nblocks = 8 * tpx;
   int rc;
   zero_u32_kernel<<<1, 64, 0, c->stream>>>(c->d_qctl, 4);
+  zero_u32_kernel<<<(c->S + 255) / 256, 256, 0, c->stream>>>(c->d_need, c->S);
   if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
   if (scw && c->Ntp > 0) {
     rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs0, rec, C, sc, scw,
-                                   c->score_thresh, c->d_pre, c->d_q, c->d_qctl, stats),
+                                   c->score_thresh, c->d_pre, c->d_q, c->d_qctl, c->d_need, stats),
                 "prebucket launch");
     if (rc) return rc;
   }
@@ -109,13 +110,16 @@ int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *r
     if (kt) kt->start();
     rc = c->hip(c->L->tp_main(c->stream, c->cfg.kepler_guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx,
                               tp, c->d_prm, xs0, rec, C, sc, nnw, scw ? c->d_pre : nullptr, c->d_q, c->d_qctl,
-                              c->d_status, stats),
+                              c->d_need, c->d_status, stats),
                 "main launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
     if (kt) kt->start();
+    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, c->d_pc, sc), "perturber cache launch");
+    if (rc) return rc;
     rc = c->hip(c->L->tp_encounter(c->stream, c->cfg.kepler_guess, enc_grid(c), 64, c->d_q, c->d_qctl,
-                                   c->d_qctl + 1, tp, c->d_prm, xs0, rec, C, sc, nnw, c->d_status, stats),
+                                   c->d_qctl + 1, tp, c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status,
+                                   stats),
                 "encounter launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);

@@ -124,3 +124,19 @@ def p9_sweep(S=4096, systems=None, tp_scale=1.0) -> mint.State:
     off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return mint.State(off, np.array(ms), np.array(xs), np.array(vs), np.concatenate(txs), np.concatenate(tvs),
                       meta=dict(workload="p9_sweep", dt=0.5, systems=len(systems)))
+
+
+def inner_belt(n_tp=20000, a_range=(1.5, 3.0), seed_key=0) -> mint.State:
+    """Sweep system 0's massive bodies with TPs on inner orbits (a in a_range
+    AU, e < 0.2, i < 10 deg).  At dt = 0.5 yr their local step tau =
+    dt sqrt(mu/r^3) is 0.08-0.3, the regime where the Kepler solve needs more
+    than one Newton iteration from the series guess."""
+    base = p9_sweep(systems=[0], tp_scale=1 / 256)
+    rng = _rng(6, seed_key)
+    a = rng.uniform(*a_range, n_tp)
+    e = rng.uniform(0.0, 0.2, n_tp)
+    inc = np.radians(rng.uniform(0.0, 10.0, n_tp))
+    ang = rng.uniform(0.0, 2 * np.pi, (3, n_tp))
+    tx, tv = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], e, a)
+    return mint.State(np.array([0, n_tp], np.int64), base.m, base.mx, base.mv, tx, tv,
+                      meta=dict(workload="inner_belt", dt=0.5))

@@ -1,0 +1,145 @@
+"""Train the learned parts on the GPU with libmint's own FP32 forward /
+backward / Adam kernels, from traces the integrator itself writes, and store
+the weights.
+
+    python train_learned.py [--out weights/learned.json] [--kepler-samples N]
+
+Kepler warm-start MLP: samples (features, c) of drifts, c = (s*/s_series - 1)
+/ tau^3 with s* the converged universal variable.  Traced from the Planet Nine
+sweep and the single-system P9 workload (their test particles and massive
+bodies) and from an inner-belt workload; only samples with tau above
+TRAIN_TAU_MIN are kept, because the library consults the network only where
+tau exceeds 0.02 (below that the series guess already converges in one Newton
+iteration).  The target is standardised for training and the scale folded back
+into the output layer.
+
+Encounter scorer: (pair features at a launch-window start, label = the exact
+check fired for that pair during the window), traced from the sweep (the trace
+keeps one negative pair in 64; negatives are then subsampled to 20 per
+positive).  The logit threshold is set for 99.5 % recall of
+positive pairs on a held-out tenth.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import mint  # noqa: E402
+import mint_workloads as W  # noqa: E402
+
+TRAIN_TAU_MIN = 0.004
+DEFAULT_OUT = os.path.join(HERE, "weights", "learned.json")
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def kepler_traces(n_target, seed):
+    feats, tgts = [], []
+    jobs = [("p9_sweep[0:512]", W.p9_sweep(systems=np.arange(512)), 400),
+            ("p9_single", W.p9_single(100_000), 400),
+            ("inner_belt", W.inner_belt(200_000, (1.2, 4.0)), 200)]
+    for name, st, steps in jobs:
+        with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES) as g:
+            g.upload(st)
+            f, t = g.trace_kepler(steps, cap=min(n_target, 8_000_000))
+        tau = 2.0 ** (4 * f[:, 0] - 6)
+        keep = (tau > TRAIN_TAU_MIN) & np.isfinite(t)
+        log(f"kepler trace {name}: {len(t)} samples, {keep.sum()} with tau > {TRAIN_TAU_MIN}")
+        feats.append(f[keep])
+        tgts.append(t[keep])
+    f = np.concatenate(feats)
+    t = np.concatenate(tgts)
+    rng = np.random.default_rng(seed)
+    if len(t) > n_target:
+        sel = rng.choice(len(t), n_target, replace=False)
+        f, t = f[sel], t[sel]
+    return f, t
+
+
+def encounter_traces(n_cap, seed):
+    st = W.p9_sweep(systems=np.arange(4096))
+    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES, chunk_steps=64) as g:
+        g.upload(st)
+        f, lab = g.trace_encounter(64 * 16, cap=n_cap)
+    pos = lab > 0.5
+    log(f"encounter trace: {len(lab)} pairs, {pos.sum()} positive")
+    rng = np.random.default_rng(seed)
+    neg_idx = np.flatnonzero(~pos)
+    keep_neg = rng.choice(neg_idx, min(len(neg_idx), 20 * max(pos.sum(), 1)), replace=False)
+    idx = np.concatenate([np.flatnonzero(pos), keep_neg])
+    rng.shuffle(idx)
+    return f[idx], lab[idx]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--kepler-samples", type=int, default=10_000_000)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=20261203)
+    args = ap.parse_args()
+    t0 = time.time()
+    report = {}
+
+    f, t = kepler_traces(args.kepler_samples, args.seed)
+    mu_t, sd_t = float(np.mean(t)), float(np.std(t) + 1e-30)
+    tn = ((t - mu_t) / sd_t).astype(np.float32)
+    with mint.Mint(dt=0.5) as g:
+        losses = g.train(mint.NET_KEPLER, f, tn, epochs=args.epochs, batch=4096, lr=2e-3, seed=args.seed)
+        w = g.nn_weights(mint.NET_KEPLER)
+    log(f"kepler MLP: {len(t)} samples, standardised loss per epoch {np.round(losses, 5).tolist()}")
+    # fold the standardisation into the output layer: y = sd * y_n + mu
+    W3 = slice(4 * 32 + 32 + 32 * 32 + 32, 4 * 32 + 32 + 32 * 32 + 32 + 32)
+    w[W3] *= sd_t
+    w[-1] = w[-1] * sd_t + mu_t
+    report["kepler"] = dict(samples=int(len(t)), target_mean=mu_t, target_std=sd_t,
+                            loss_standardised=[float(x) for x in losses], tau_min=TRAIN_TAU_MIN)
+    wk = w
+
+    fs, ls = encounter_traces(4_000_000, args.seed)
+    nval = len(ls) // 10
+    with mint.Mint(dt=0.5) as g:
+        losses = g.train(mint.NET_SCORER, fs[nval:], ls[nval:], epochs=30, batch=2048, lr=3e-3, seed=args.seed)
+        ws = g.nn_weights(mint.NET_SCORER)
+        logit = g.nn_forward(mint.NET_SCORER, fs[:nval])
+    pos = ls[:nval] > 0.5
+    thr = float(np.quantile(logit[pos], 0.005)) if pos.any() else 0.0
+    recall = float(np.mean(logit[pos] > thr)) if pos.any() else 0.0
+    precision = float(np.mean(pos[logit > thr])) if np.any(logit > thr) else 0.0
+    log(f"scorer: {len(ls)} pairs, loss per epoch {np.round(losses[-5:], 4).tolist()}, threshold {thr:.3f}, "
+        f"held-out recall {recall:.4f}, precision {precision:.4f} (negatives subsampled 20:1)")
+    report["scorer"] = dict(pairs=int(len(ls)), positives=int(np.sum(ls > 0.5)), threshold=thr,
+                            heldout_recall=recall, heldout_precision_subsampled=precision,
+                            loss=[float(x) for x in losses])
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as fh:
+        json.dump(dict(kepler=[float(x) for x in wk], scorer=[float(x) for x in ws], score_threshold=thr,
+                       report=report, seconds=time.time() - t0), fh)
+    log(f"wrote {args.out} in {time.time() - t0:.1f} s")
+
+
+def load(path=DEFAULT_OUT):
+    with open(path) as fh:
+        d = json.load(fh)
+    return np.array(d["kepler"], np.float32), np.array(d["scorer"], np.float32), float(d["score_threshold"])
+
+
+def apply(g, path=DEFAULT_OUT):
+    """Load stored weights into a context (both networks and the threshold)."""
+    wk, ws, thr = load(path)
+    g.set_nn_weights(mint.NET_KEPLER, wk)
+    g.set_nn_weights(mint.NET_SCORER, ws)
+    g.set_score_threshold(thr)
+    return thr
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,99 @@
+"""Learned parts on the GPU: the FP32 forward/backward kernels against the
+FP64 reference (oracle/nn_oracle.py), training, and the two result-neutrality
+guarantees (the exact Newton tolerance and the exact distance check decide),
+plus what the trained Kepler network gains over the series guess."""
+import os
+
+import numpy as np
+import pytest
+
+import mint
+import mint_workloads as W
+import nn_oracle as N
+import oracle as O
+import train_learned as T
+from gpu_cases import encounter_system, run_gpu
+
+pytestmark = pytest.mark.gpu
+HAVE_WEIGHTS = os.path.exists(T.DEFAULT_OUT)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_gpu_gradients_match_fp64_reference(which):
+    rng = np.random.default_rng(11 + which)
+    w = rng.normal(scale=0.3, size=N.nparams(which)).astype(np.float32)
+    X = rng.normal(size=(3000, 4 if which == 0 else 6)).astype(np.float32)
+    t = (rng.normal(size=3000) if which == 0 else (rng.uniform(size=3000) < 0.3)).astype(np.float32)
+    with mint.Mint() as g:
+        g.set_nn_weights(which, w)
+        loss, grad = g.nn_loss_grad(which, X, t)
+        out = g.nn_forward(which, X[:100])
+    ref = N.grad(which, w.astype(np.float64), X.astype(np.float64), t.astype(np.float64))
+    assert loss == pytest.approx(N.loss(which, w, X, t), rel=1e-4)
+    assert np.max(np.abs(grad - ref)) <= 2e-4 * np.max(np.abs(ref))
+    assert np.allclose(out, N.forward(which, w, X[:100]), rtol=1e-4, atol=1e-5)
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, size=(50000, 4)).astype(np.float32)
+    t = (np.sin(2 * X[:, 0]) * X[:, 1] + 0.5 * X[:, 2] ** 2 - X[:, 3]).astype(np.float32)
+    with mint.Mint() as g:
+        losses = g.train(mint.NET_KEPLER, X, t, epochs=8, batch=1024, lr=3e-3, seed=5)
+    assert losses[-1] < 0.1 * losses[0], losses
+
+
+def _weights(g):
+    if not HAVE_WEIGHTS:
+        pytest.skip("no trained weights (run p9-integrator_amd/train_learned.py on the GPU)")
+    return T.apply(g)
+
+
+def test_learned_guess_is_bitwise_neutral_on_the_sweep():
+    st = W.p9_sweep(systems=np.arange(12), tp_scale=1 / 8)
+    outs = []
+    for guess in (mint.GUESS_SERIES, mint.GUESS_LEARNED):
+        with mint.Mint(dt=0.5, kepler_guess=guess, chunk_steps=32, block_size=128, tps_per_lane=1) as g:
+            _weights(g)
+            g.upload(st)
+            g.step(100)
+            outs.append(g.download())
+    assert np.array_equal(outs[0].tx, outs[1].tx) and np.array_equal(outs[0].tv, outs[1].tv)
+
+
+def test_learned_guess_agrees_where_it_engages_and_saves_newton_iterations():
+    st = W.inner_belt(4000)
+    res = {}
+    for guess in (mint.GUESS_SERIES, mint.GUESS_LEARNED):
+        with mint.Mint(dt=0.5, kepler_guess=guess, collect_stats=1, chunk_steps=50, block_size=128,
+                       tps_per_lane=1) as g:
+            _weights(g)
+            g.upload(st)
+            g.step(100)
+            res[guess] = (g.download(), g.stats())
+    a, sa = res[mint.GUESS_SERIES]
+    b, sb = res[mint.GUESS_LEARNED]
+    ref, _, _ = O.run(st, 100, 0.5, guess=1)
+    rel = np.max(np.abs(b.tx - ref["tx"]) / np.abs(ref["tx"]).max(axis=1, keepdims=True))
+    assert rel <= 1e-12, rel
+    assert np.array_equal(a.tx, ref["tx"])
+    it_series = sa["newton_iters"] / sa["kepler_solves"]
+    it_learned = sb["newton_iters"] / sb["kepler_solves"]
+    print(f"mean Newton iterations per drift: series guess {it_series:.4f}, learned guess {it_learned:.4f}")
+    assert it_learned < it_series
+
+
+def test_scorer_prebucketing_is_bitwise_neutral():
+    st = encounter_system(n_tp=400, seed=9)
+    outs = []
+    stats = []
+    for scorer in (0, 1):
+        with mint.Mint(dt=0.5, use_scorer=scorer, collect_stats=1, chunk_steps=20, block_size=64,
+                       tps_per_lane=1) as g:
+            _weights(g)
+            g.upload(st)
+            g.step(60)
+            outs.append(g.download())
+            stats.append(g.stats())
+    assert stats[1]["prebucketed"] > 0
+    assert np.array_equal(outs[0].tx, outs[1].tx) and np.array_equal(outs[0].tv, outs[1].tv)
