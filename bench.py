@@ -45,7 +45,35 @@ def log(msg):
 
 
 def make_ctx(form, guess, scorer, stats=0, **kw):
-    return mint.Mint(dt=0.5, form=form, kepler_guess=guess, use_scorer=scorer, collect_stats=stats, **kw)
+    g = mint.Mint(dt=0.5, form=form, kepler_guess=guess, use_scorer=scorer, collect_stats=stats, **kw)
+    if guess == mint.GUESS_LEARNED or scorer:
+        import train_learned as T
+
+        if not os.path.exists(T.DEFAULT_OUT):
+            log("bench: no trained weights; training them on this GPU first (untimed)")
+            sys.argv = [sys.argv[0]]
+            T.main()
+        T.apply(g)
+    return g
+
+
+def energy_gate(st, E0, E1, mx, mv, steps):
+    """|dE/E| of the massive bodies over the run, and the GPU's energy against
+    the CPU oracle's for the same steps (massive bodies only: TPs are
+    massless and do not enter the energy)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # test infrastructure: checker only
+
+    ms = mint.State(np.zeros(st.n_systems + 1, np.int64), st.m, st.mx, st.mv, np.zeros((0, 3)), np.zeros((0, 3)))
+    t0 = time.perf_counter()
+    out, _, _ = O.run(ms, steps, 0.5, guess=1, threads=min(16, os.cpu_count() or 1))
+    Eo = O.energy(st.m, out["mx"], out["mv"])
+    Eg = O.energy(st.m, mx, mv)
+    return {"max_rel_dE_over_run": float(np.max(np.abs((E1 - E0) / E0))),
+            "max_rel_E_gpu_minus_oracle": float(np.max(np.abs((Eg - Eo) / E0))),
+            "massive_state_bitwise_equal_to_oracle": bool(np.array_equal(out["mx"], mx) and
+                                                          np.array_equal(out["mv"], mv)),
+            "oracle_seconds": time.perf_counter() - t0}
 
 
 def timed_run(g, st, steps, warmup):
@@ -104,7 +132,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=64)
-    ap.add_argument("--form", default="batched", choices=["learned", "batched", "flat", "plain"])
+    ap.add_argument("--form", default="learned", choices=["learned", "batched", "flat", "plain"])
     ap.add_argument("--systems", type=int, default=4096)
     ap.add_argument("--no-baseline", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-ablation", action="store_true", help="skip the plain / flat comparison runs")
@@ -141,9 +169,17 @@ def main():
     if form == mint.FORM_BATCHED and not shape:
         g.upload(st)
         shape = g.autotune(args.tune_cache)
+        if scorer:
+            # the scorer is trained for one launch window; keep the tuned block / TPs per lane
+            import train_learned as T
+
+            shape = (shape[0], shape[1], T.window())
+            g.close()
+            g = make_ctx(form, guess, scorer, block_size=shape[0], tps_per_lane=shape[1], chunk_steps=shape[2])
         log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]}")
     wall, ev_s, E0, E1 = timed_run(g, st, args.steps, args.warmup)
     status = g.status()
+    fin = g.download()
     value = ntp * args.steps / ev_s
     log(f"bench: {args.form}: {args.steps} steps in {ev_s * 1e3:.1f} ms (events) / {wall * 1e3:.1f} ms (host) "
         f"-> {value:.4e} TP-steps/s")
@@ -175,6 +211,12 @@ def main():
         gp.close()
 
     forms = {args.form: value}
+    if not args.no_ablation and args.form == "learned":
+        gb = make_ctx(mint.FORM_BATCHED, mint.GUESS_SERIES, 0, block_size=shape[0], tps_per_lane=shape[1],
+                      chunk_steps=shape[2])
+        _, be, _, _ = timed_run(gb, st, args.steps, args.warmup)
+        forms["batched"] = ntp * args.steps / be
+        gb.close()
     if not args.no_ablation and form == mint.FORM_BATCHED:
         # flat: one launch per step over all systems
         gf = make_ctx(mint.FORM_FLAT, mint.GUESS_SERIES, 0)
@@ -215,7 +257,8 @@ def main():
         "cpu_baseline": base,
         "forms_tp_steps_per_s": forms,
         "speedup_vs_plain": value / forms["plain"] if "plain" in forms else None,
-        "energy": {"max_rel_dE_over_run": dE, "status_bits_or": int(np.bitwise_or.reduce(status))},
+        "energy": dict(energy_gate(st, E0, E1, fin.mx, fin.mv, args.steps) if not args.no_baseline else
+                       {"max_rel_dE_over_run": dE}, status_bits_or=int(np.bitwise_or.reduce(status))),
         "host_wall_s": wall,
     }
     out.update(extra)

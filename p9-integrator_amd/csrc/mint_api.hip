@@ -269,7 +269,7 @@ int mint_create(const mint_config *cfg, void **out) {
     return MINT_EINVAL;
   }
   if (hipSetDevice(k.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipStreamCreate(&c->stream2) != hipSuccess) {
+      hipStreamCreate(&c->stream2) != hipSuccess || hipStreamCreate(&c->stream3) != hipSuccess) {
     g_create_error = "mint_create: cannot initialise device";
     delete c;
     return MINT_EHIP;
@@ -287,12 +287,15 @@ void mint_destroy(void *ctx) {
   hipStreamSynchronize(c->stream);
   c->free_device();
   hipStreamSynchronize(c->stream2);
+  hipStreamSynchronize(c->stream3);
+  for (hipEvent_t e : c->ev_side) hipEventDestroy(e);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->ev_eph) hipEventDestroy(e);
   for (hipEvent_t e : c->ev_tp) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
   hipStreamDestroy(c->stream2);
+  hipStreamDestroy(c->stream3);
   delete c;
 }
 

@@ -23,6 +23,8 @@ struct Ctx {
   std::string err;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // ephemeris runs one window ahead on this stream
+  hipStream_t stream3 = nullptr;   // pre-bucketed encounter work beside the main kernel
+  std::vector<hipEvent_t> ev_side;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> ev_eph, ev_tp;  // per-window completion events of the pipeline
 
@@ -47,7 +49,7 @@ struct Ctx {
   int ntiles = 0;
   long long tiles_key = -1;
   QEntry *d_q = nullptr;
-  unsigned int *d_qctl = nullptr;  // [0] count, [1] head
+  unsigned int *d_qctl = nullptr;  // [0] entries so far, [2] entries from the pre-bucketing pass
   unsigned int *d_status = nullptr;
   unsigned long long *d_stats = nullptr;
   float *d_nnw_kep = nullptr, *d_nnw_sc = nullptr;

@@ -94,8 +94,9 @@ __device__ __forceinline__ double tp_guess(const KepStart &k, double mu, double 
   if (GUESS == 0) return guess_crude(k, dt);
   double s = guess_series(k, dt);
   if (GUESS == 2) {
-    const double tau0 = dt * sqrt(mu / (k.r0 * (k.r0 * k.r0)));
-    if (tau0 > KM_TAU_GATE) {
+    // gate tau = dt sqrt(mu / r0^3) > KM_TAU_GATE, tested without sqrt or division
+    const double ir3 = k.ir0 * (k.ir0 * k.ir0);
+    if ((dt * dt) * (mu * ir3) > KM_TAU_GATE * KM_TAU_GATE) {
       float f[4];
       double tau;
       kepler_features(k, mu, dt, f, tau);
@@ -211,12 +212,13 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
 // ---------------------------------------------------------------------------
 // Perturber cache: one lane per (system, step, substep point, body), each a
 // Kepler drift of the body's drift-start state (Y, W) from the ephemeris.
-// Only systems flagged in need[] (a TP of theirs entered the encounter queue
-// in this window) are computed.
+// Only systems with need[s] == want are computed: 1 = a TP of theirs was
+// pre-bucketed at the window start, 2 = a TP entered the queue mid-window.
 template <int NM>
 __global__ void __launch_bounds__(256) pcache_kernel(int s_begin, int s_end, int C,
                                                      const double *__restrict__ rec,
                                                      const unsigned int *__restrict__ need,
+                                                     unsigned int want,
                                                      double *__restrict__ pc, StepConst sc) {
   constexpr int REC = rec_size(NM);
   const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NM;
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(256) pcache_kernel(int s_begin, int s_end, int
   u /= PC_PTS;
   const int k = (int)(u % C);
   const int s = s_begin + (int)(u / C);
-  if (need[s] == 0u) return;
+  if (need[s] != want) return;
   const double *r = rec + ((size_t)s * C + k) * REC;
   double px = r[R_Y(NM) + 3 * j], py = r[R_Y(NM) + 3 * j + 1], pz = r[R_Y(NM) + 3 * j + 2];
   double qx = r[R_W(NM) + 3 * j], qy = r[R_W(NM) + 3 * j + 1], qz = r[R_W(NM) + 3 * j + 2];
@@ -398,7 +400,7 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
       if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
         const unsigned int slot = atomicAdd(qcount, 1u);
         q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
-        need[s] = 1u;
+        if (need[s] == 0u) need[s] = 2u;
         act[p] = false;
         ++n_defer;
         n_idle += (unsigned)(C - k);
@@ -488,7 +490,7 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
 template <int NM, int GUESS>
 __global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
-    unsigned int *__restrict__ qhead, TPArrays tp, const double *__restrict__ prm_all,
+    const unsigned int *__restrict__ qbegin, TPArrays tp, const double *__restrict__ prm_all,
     const double *__restrict__ xs0, const double *__restrict__ rec, const double *__restrict__ pc,
     int C, StepConst sc, const float *__restrict__ nnw, unsigned int *__restrict__ status,
     unsigned long long *__restrict__ stats) {
@@ -496,16 +498,18 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   // BS extrapolation tableau of each lane, lane-interleaved in LDS (one-wave blocks)
   __shared__ double Rsh[KMAX * 6 * 64];
   double *Rt = Rsh + (threadIdx.x & 63u);
-  const unsigned int n = *qcount;
+  // entries [qbegin, qcount) (qbegin null: from 0)
+  const unsigned int e0 = qbegin ? *qbegin : 0u;
+  const unsigned int e1 = *qcount;
+  const unsigned int n = e1 > e0 ? e1 - e0 : 0u;
   unsigned int n_kep = 0, n_newton = 0, n_bs = 0, n_rhs = 0, n_step = 0;
   const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
   const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned int lane = threadIdx.x & 63u;
   unsigned int L = (n + nw - 1u) / nw;
   L = L < 1u ? 1u : (L > 64u ? 64u : L);
-  (void)qhead;
   for (unsigned int e = w * L + lane; lane < L && e < n; e += nw * L) {
-    const QEntry en = q[e];
+    const QEntry en = q[e0 + e];
     const int s = en.sys;
     const double *prm = prm_all + (size_t)s * prm_size(NM);
     const double *rsys = rec + (size_t)s * C * REC;
@@ -574,6 +578,8 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
     wave_add(stats + S_ENCSTEPS, n_step);
   }
 }
+
+static __global__ void copy_u32_kernel(unsigned int *dst, const unsigned int *src) { *dst = *src; }
 
 static __global__ void zero_u32_kernel(unsigned int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

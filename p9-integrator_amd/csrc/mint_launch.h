@@ -22,7 +22,7 @@ struct Launchers {
                         QEntry *q, unsigned int *qcount, unsigned int *need,
                         unsigned int *status, unsigned long long *stats);
   hipError_t (*tp_encounter)(hipStream_t st, int guess, int grid, int block, const QEntry *q,
-                             const unsigned int *qcount, unsigned int *qhead, TPArrays tp,
+                             const unsigned int *qcount, const unsigned int *qbegin, TPArrays tp,
                              const double *prm, const double *xs0, const double *rec,
                              const double *pc, int C, StepConst sc, const float *nnw,
                              unsigned int *status, unsigned long long *stats);
@@ -32,7 +32,7 @@ struct Launchers {
                              QEntry *q, unsigned int *qcount, unsigned int *need,
                              unsigned long long *stats);
   hipError_t (*pcache)(hipStream_t st, int s_begin, int s_end, int C, const double *rec,
-                       const unsigned int *need, double *pc, StepConst sc);
+                       const unsigned int *need, unsigned int want, double *pc, StepConst sc);
 };
 
 const Launchers *launchers_nm1();

@@ -77,22 +77,22 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
 }
 
 hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
-                          const unsigned int *qcount, unsigned int *qhead, TPArrays tp,
+                          const unsigned int *qcount, const unsigned int *qbegin, TPArrays tp,
                           const double *prm, const double *xs0, const double *rec,
                           const double *pc, int C, StepConst sc, const float *nnw,
                           unsigned int *status, unsigned long long *stats) {
   if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
   switch (guess) {
     case 0:
-      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     case 1:
-      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     case 2:
-      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qhead, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     default:
@@ -113,10 +113,11 @@ hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArra
 }
 
 hipError_t pcache_l(hipStream_t st, int s_begin, int s_end, int C, const double *rec,
-                    const unsigned int *need, double *pc, StepConst sc) {
+                    const unsigned int *need, unsigned int want, double *pc, StepConst sc) {
   const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NMV;
   if (n <= 0) return hipSuccess;
-  pcache_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(s_begin, s_end, C, rec, need, pc, sc);
+  pcache_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(s_begin, s_end, C, rec, need, want, pc,
+                                                                  sc);
   return hipGetLastError();
 }
 

@@ -86,8 +86,13 @@ int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, KTimer *
   return MINT_OK;
 }
 
-// TP side of one launch window of C steps (queue reset, optional learned
-// pre-bucketing, main kernel, encounter queue) on the context stream.
+hipEvent_t window_event(std::vector<hipEvent_t> &v, size_t i);
+
+// TP side of one launch window of C steps on the context stream: queue reset,
+// then (with the scorer) the pre-bucketing pass whose entries are carried by
+// an encounter kernel on the side stream while the main kernel runs, then the
+// main kernel, the perturber cache of systems that deferred mid-window, and
+// the encounter kernel over those deferrals.
 int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *rec, KTimer *kt) {
   const StepConst sc = c->step_const();
   const TPArrays tp = c->tp_arrays();
@@ -96,33 +101,59 @@ int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *r
   const float *scw = c->cfg.use_scorer ? c->d_nnw_sc : nullptr;
   const int tpx = (c->ntiles + 7) / 8;
   const int nblocks = 8 * tpx;
+  const int guess = c->cfg.kepler_guess;
   int rc;
   zero_u32_kernel<<<1, 64, 0, c->stream>>>(c->d_qctl, 4);
   zero_u32_kernel<<<(c->S + 255) / 256, 256, 0, c->stream>>>(c->d_need, c->S);
   if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
-  if (scw && c->Ntp > 0) {
+  const bool side = scw && c->Ntp > 0;
+  if (side) {
     rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs0, rec, C, sc, scw,
                                    c->score_thresh, c->d_pre, c->d_q, c->d_qctl, c->d_need, stats),
                 "prebucket launch");
     if (rc) return rc;
+    copy_u32_kernel<<<1, 1, 0, c->stream>>>(c->d_qctl + 2, c->d_qctl);
+    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, 1u, c->d_pc, sc), "perturber cache launch");
+    if (rc) return rc;
+    if (kt) {
+      kt->start();
+      rc = c->hip(c->L->tp_encounter(c->stream, guess, enc_grid(c), 64, c->d_q, c->d_qctl + 2, nullptr, tp,
+                                     c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status, stats),
+                  "encounter launch");
+      if (rc) return rc;
+      kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
+    } else {
+      hipEvent_t f = window_event(c->ev_side, 0);
+      if ((rc = c->hip(hipEventRecord(f, c->stream), "side fork"))) return rc;
+      if ((rc = c->hip(hipStreamWaitEvent(c->stream3, f, 0), "side wait"))) return rc;
+      rc = c->hip(c->L->tp_encounter(c->stream3, guess, enc_grid(c), 64, c->d_q, c->d_qctl + 2, nullptr, tp,
+                                     c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status, stats),
+                  "encounter launch");
+      if (rc) return rc;
+    }
   }
   if (c->ntiles > 0) {
     if (kt) kt->start();
-    rc = c->hip(c->L->tp_main(c->stream, c->cfg.kepler_guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx,
-                              tp, c->d_prm, xs0, rec, C, sc, nnw, scw ? c->d_pre : nullptr, c->d_q, c->d_qctl,
-                              c->d_need, c->d_status, stats),
+    rc = c->hip(c->L->tp_main(c->stream, guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx, tp, c->d_prm,
+                              xs0, rec, C, sc, nnw, scw ? c->d_pre : nullptr, c->d_q, c->d_qctl, c->d_need,
+                              c->d_status, stats),
                 "main launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
     if (kt) kt->start();
-    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, c->d_pc, sc), "perturber cache launch");
+    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, 2u, c->d_pc, sc), "perturber cache launch");
     if (rc) return rc;
-    rc = c->hip(c->L->tp_encounter(c->stream, c->cfg.kepler_guess, enc_grid(c), 64, c->d_q, c->d_qctl,
-                                   c->d_qctl + 1, tp, c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status,
-                                   stats),
+    rc = c->hip(c->L->tp_encounter(c->stream, guess, enc_grid(c), 64, c->d_q, c->d_qctl,
+                                   side ? c->d_qctl + 2 : nullptr, tp, c->d_prm, xs0, rec, c->d_pc, C, sc, nnw,
+                                   c->d_status, stats),
                 "encounter launch");
     if (rc) return rc;
     if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
+  }
+  if (side && !kt) {
+    hipEvent_t j = window_event(c->ev_side, 1);
+    if ((rc = c->hip(hipEventRecord(j, c->stream3), "side join"))) return rc;
+    if ((rc = c->hip(hipStreamWaitEvent(c->stream, j, 0), "side join wait"))) return rc;
   }
   return MINT_OK;
 }

@@ -16,8 +16,9 @@ into the output layer.
 Encounter scorer: (pair features at a launch-window start, label = the exact
 check fired for that pair during the window), traced from the sweep (the trace
 keeps one negative pair in 64; negatives are then subsampled to 20 per
-positive).  The logit threshold is set for 99.5 % recall of
-positive pairs on a held-out tenth.
+positive).  The logit threshold is set for 99.8 % recall of
+positive pairs on a held-out tenth.  The scorer's approach features scale with
+the launch window, so it is traced at, and used with, WINDOW steps.
 """
 import argparse
 import json
@@ -64,11 +65,14 @@ def kepler_traces(n_target, seed):
     return f, t
 
 
+WINDOW = 128  # launch window (steps) the scorer is trained for; its features scale with it
+
+
 def encounter_traces(n_cap, seed):
     st = W.p9_sweep(systems=np.arange(4096))
-    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES, chunk_steps=64) as g:
+    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES, chunk_steps=WINDOW) as g:
         g.upload(st)
-        f, lab = g.trace_encounter(64 * 16, cap=n_cap)
+        f, lab = g.trace_encounter(WINDOW * 8, cap=n_cap)
     pos = lab > 0.5
     log(f"encounter trace: {len(lab)} pairs, {pos.sum()} positive")
     rng = np.random.default_rng(seed)
@@ -111,7 +115,7 @@ def main():
         ws = g.nn_weights(mint.NET_SCORER)
         logit = g.nn_forward(mint.NET_SCORER, fs[:nval])
     pos = ls[:nval] > 0.5
-    thr = float(np.quantile(logit[pos], 0.005)) if pos.any() else 0.0
+    thr = float(np.quantile(logit[pos], 0.002)) if pos.any() else 0.0
     recall = float(np.mean(logit[pos] > thr)) if pos.any() else 0.0
     precision = float(np.mean(pos[logit > thr])) if np.any(logit > thr) else 0.0
     log(f"scorer: {len(ls)} pairs, loss per epoch {np.round(losses[-5:], 4).tolist()}, threshold {thr:.3f}, "
@@ -122,7 +126,7 @@ def main():
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as fh:
         json.dump(dict(kepler=[float(x) for x in wk], scorer=[float(x) for x in ws], score_threshold=thr,
-                       report=report, seconds=time.time() - t0), fh)
+                       window=WINDOW, report=report, seconds=time.time() - t0), fh)
     log(f"wrote {args.out} in {time.time() - t0:.1f} s")
 
 
@@ -130,6 +134,12 @@ def load(path=DEFAULT_OUT):
     with open(path) as fh:
         d = json.load(fh)
     return np.array(d["kepler"], np.float32), np.array(d["scorer"], np.float32), float(d["score_threshold"])
+
+
+def window(path=DEFAULT_OUT):
+    """Launch window the scorer was trained for."""
+    with open(path) as fh:
+        return int(json.load(fh).get("window", 64))
 
 
 def apply(g, path=DEFAULT_OUT):
