@@ -30,6 +30,10 @@ METRIC = "test-particle steps/s, 4096-system P9 sweep, learned+batched vs plain,
 FP64_PEAK = 78.6e12   # MI355X dense FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12     # B/s
 N_M = 5
+# HBM bytes per tp_main_kernel launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (calibrated by a 1 GiB copy: FETCH x2.0, WRITE x1.0), per launch shape
+# (block, TPs per lane, window); profiles/tp_main_kernel_p9sweep4096.md
+MEASURED_TRAFFIC = {(128, 1, 128): 910.8e6}
 
 
 def flops_per_tp_step(nbar: float, nm: int = N_M) -> float:
@@ -203,7 +207,7 @@ def main():
         avg_main_ms = t["ms_tp_main"] / max(t["n_tp_main"], 1)
         achieved = F * main_steps / (t["ms_tp_main"] / 1e3)
         roof = {"bound": "fp64", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK, "traffic": None,
+                "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(tuple(shape)),
                 "kernel": "tp_main_kernel", "flop_per_tp_step": F, "newton_mean": nbar,
                 "tp_steps_per_launch": main_steps / max(t["n_tp_main"], 1), "avg_launch_ms": avg_main_ms}
         extra["profile"] = {"steps": nprof, "stats": s, "timing": t,
