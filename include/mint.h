@@ -148,6 +148,17 @@ int mint_step(void *ctx, int64_t nsteps);
  * Only the non-NULL arrays of st are written; shapes must match the upload. */
 int mint_download(void *ctx, mint_state *st);
 
+/* Binary checkpoint of the device state, bit for bit: the internal
+ * (democratic-heliocentric) state, masses, changeover radii, the CSR layout,
+ * the step parameters and `step_index`.  Resuming from it continues exactly
+ * as if the run had not stopped.  The reference persists osculating elements
+ * every 20 launches (src/mercurius.cu:201-249) but never reads them back, and
+ * its element round trip is lossy (src/sim.cuh:255-256). */
+int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index);
+/* Replaces the context's state (and dt, central mass, softening, BS
+ * tolerance) by the checkpoint's; *step_index receives the saved index. */
+int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index);
+
 /* Total energy of each system's massive bodies (democratic-heliocentric
  * Hamiltonian; the reference has no energy diagnostic). E[S]. */
 int mint_energy(void *ctx, double *E);

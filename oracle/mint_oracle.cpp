@@ -63,8 +63,11 @@ void stumpff(double z, double c[4]) {
     z *= 0.25;
     n++;
   }
-  double s2 = C2[6], s3 = C3[6];
-  for (int k = 5; k >= 0; --k) {
+  // truncated by |z| band: degree 2 (|z| <= 1e-5), 3 (<= 1e-3), else 6
+  const double az = std::fabs(z);
+  const int top = az <= 1e-5 ? 2 : (az <= 1e-3 ? 3 : 6);
+  double s2 = C2[top], s3 = C3[top];
+  for (int k = top - 1; k >= 0; --k) {
     s2 = std::fma(s2, z, C2[k]);
     s3 = std::fma(s3, z, C3[k]);
   }
@@ -169,15 +172,42 @@ struct Rc {
 };
 Rc make_rc(double rc) { return {0.1 * rc, 1.0 / (0.9 * rc), rc * rc}; }
 
-// K m / ((r^2+eps^2) r), single division
-double kick_w(double r2, double eps2, double m, const Rc &c) {
+// K m / ((r^2+eps^2) r) as a numerator / denominator pair
+void kick_nd(double r2, double eps2, double m, const Rc &c, double &num, double &den) {
   const double r = std::sqrt(r2);
   const double D = (r2 + eps2) * r;
   const double y = (r - c.lo) * c.winv;
-  if (y <= 0.0) return 0.0 / D;
-  if (y >= 1.0) return m / D;
-  const double y2 = y * y;
-  return (m * y2) / (D * std::fma(2.0, y2, std::fma(-2.0, y, 1.0)));
+  num = m;
+  den = D;
+  if (y <= 0.0) {
+    num = 0.0;
+  } else if (y < 1.0) {
+    const double y2 = y * y;
+    num = m * y2;
+    den = D * std::fma(2.0, y2, std::fma(-2.0, y, 1.0));
+  }
+}
+
+double kick_w(double r2, double eps2, double m, const Rc &c) {
+  double num, den;
+  kick_nd(r2, eps2, m, c, num, den);
+  return num / den;
+}
+
+// num[i] / den[i] for all i with one division: prefix products P, inv = 1/P_last,
+// walk back w_i = num_i (inv P_{i-1}), inv *= den_i, w_0 = num_0 inv
+constexpr int MAXB = 8;  // massive bodies per system
+
+void batched_div(int n, const double *num, const double *den, double *w) {
+  double P[MAXB];
+  P[0] = den[0];
+  for (int i = 1; i < n; ++i) P[i] = P[i - 1] * den[i];
+  double inv = 1.0 / P[n - 1];
+  for (int i = n - 1; i >= 1; --i) {
+    w[i] = num[i] * (inv * P[i - 1]);
+    inv = inv * den[i];
+  }
+  w[0] = num[0] * inv;
 }
 
 // (1-K) m / ((r^2+eps^2) r)
@@ -202,13 +232,18 @@ struct Sys {
 
 // kick acceleration of a TP (body_interaction_kick, src/sim.cuh:327-382)
 V3 accel_tp(const V3 &x, const std::vector<V3> &X, const Sys &s, double eps2) {
+  V3 d[MAXB];
+  double num[MAXB], den[MAXB], w[MAXB];
+  for (int j = 0; j < s.nm; ++j) {
+    d[j] = sub(X[j], x);
+    kick_nd(dot(d[j], d[j]), eps2, s.m[j], s.rc_tp[j], num[j], den[j]);
+  }
+  batched_div(s.nm, num, den, w);
   V3 a{0, 0, 0};
   for (int j = 0; j < s.nm; ++j) {
-    const V3 d = sub(X[j], x);
-    const double w = kick_w(dot(d, d), eps2, s.m[j], s.rc_tp[j]);
-    a.x = std::fma(w, d.x, a.x);
-    a.y = std::fma(w, d.y, a.y);
-    a.z = std::fma(w, d.z, a.z);
+    a.x = std::fma(w[j], d[j].x, a.x);
+    a.y = std::fma(w[j], d[j].y, a.y);
+    a.z = std::fma(w[j], d[j].z, a.z);
   }
   return a;
 }
@@ -217,13 +252,22 @@ std::vector<V3> accel_massive(const std::vector<V3> &X, const Sys &s, double eps
   std::vector<V3> A(s.nm);
   for (int j = 0; j < s.nm; ++j) {
     V3 a{0, 0, 0};
-    for (int k = 0; k < s.nm; ++k) {
-      if (k == j) continue;
-      const V3 d = sub(X[k], X[j]);
-      const double w = kick_w(dot(d, d), eps2, s.m[k], s.rc_pair[j * s.nm + k]);
-      a.x = std::fma(w, d.x, a.x);
-      a.y = std::fma(w, d.y, a.y);
-      a.z = std::fma(w, d.z, a.z);
+    if (s.nm > 1) {
+      V3 d[MAXB];
+      double num[MAXB] = {}, den[MAXB] = {}, w[MAXB] = {};
+      int q = 0;
+      for (int k = 0; k < s.nm; ++k) {
+        if (k == j) continue;
+        d[q] = sub(X[k], X[j]);
+        kick_nd(dot(d[q], d[q]), eps2, s.m[k], s.rc_pair[j * s.nm + k], num[q], den[q]);
+        ++q;
+      }
+      batched_div(q, num, den, w);
+      for (int p = 0; p < q; ++p) {
+        a.x = std::fma(w[p], d[p].x, a.x);
+        a.y = std::fma(w[p], d[p].y, a.y);
+        a.z = std::fma(w[p], d[p].z, a.z);
+      }
     }
     A[j] = a;
   }

@@ -1,6 +1,7 @@
 // mint_cli.cpp — the `mint` command line tool on top of libmint.
 //
 //   mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]
+//        [-s checkpoint_out] [-r checkpoint_in]
 //
 // Same flags and JSON schema as the reference driver (README.md:10-19;
 // args_parse src/simutils.cuh:138-175; sim_from_config_file
@@ -14,6 +15,9 @@
 //   optional keys: "dt" (default 0.8219, src/mercurius.cu:43), "n_hill",
 //      "bs_tol", "softening2", "central_mass", "form" (plain|flat|batched),
 //      "checkpoint_every" (steps between output checkpoints; default: end only).
+// -s / -r (additions): write a bit-exact binary checkpoint at the end of the
+// run / resume from one (the state replaces the configuration's initial state;
+// -c still names the bodies for the output files).
 // Output files keep the reference formats: positions appended to output_<dev>
 // (src/simutils.cuh:186-230) and elements in elements_<dev>.txt
 // (src/mercurius.cu:210-248; src/simutils.cuh:177-184).
@@ -51,14 +55,15 @@ Body body_from(const jsonmin::Value &v, const std::string &dflt_name) {
 }
 
 struct Args {
-  std::string config, output;
+  std::string config, output, save, resume;
   long long steps = 1;
   int device = 0;
   bool info = false, print_positions = false;
 };
 
 int usage() {
-  std::cerr << "usage: mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]\n";
+  std::cerr << "usage: mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p] "
+               "[-s checkpoint_out] [-r checkpoint_in]\n";
   return 2;
 }
 
@@ -76,10 +81,12 @@ bool parse_args(int argc, char **argv, Args &a) {
       a.info = true;
     } else if (f == "-p") {
       a.print_positions = true;
-    } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d") {
-      const char *v = need(f == "-c" ? "a file" : f == "-o" ? "a file" : "a number");
+    } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d" || f == "-s" || f == "-r") {
+      const char *v = need(f == "-t" || f == "-d" ? "a number" : "a file");
       if (!v) return false;
       if (f == "-c") a.config = v;
+      if (f == "-s") a.save = v;
+      if (f == "-r") a.resume = v;
       if (f == "-o") a.output = v;
       if (f == "-t") a.steps = std::atoll(v);
       if (f == "-d") a.device = std::atoi(v);
@@ -204,6 +211,11 @@ int main(int argc, char **argv) {
   mint_state st{S, NM, off.data(), m.data(), mx.data(), mv.data(), t[0].data(), t[1].data(), t[2].data(),
                 t[3].data(), t[4].data(), t[5].data(), nullptr};
   check(mint_upload(ctx, &st), ctx, "upload");
+  int64_t step0 = 0;
+  if (!args.resume.empty()) {
+    check(mint_load_checkpoint(ctx, args.resume.c_str(), &step0), ctx, "resume");
+    if (args.info) std::cout << "Resumed from " << args.resume << " at step " << step0 << "\n";
+  }
   std::vector<double> E0(S), E1(S);
   check(mint_energy(ctx, E0.data()), ctx, "energy");
   if (args.info) {
@@ -274,6 +286,7 @@ int main(int argc, char **argv) {
       ef << "\n";
     }
   }
+  if (!args.save.empty()) check(mint_save_checkpoint(ctx, args.save.c_str(), step0 + done), ctx, "checkpoint");
   check(mint_energy(ctx, E1.data()), ctx, "energy");
   std::vector<uint32_t> status(S);
   check(mint_status(ctx, status.data()), ctx, "status");

@@ -490,6 +490,105 @@ int mint_download(void *ctx, mint_state *st) {
   return MINT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoint file: magic, version, S, NM, Ntp, step index, dt, M, n_hill,
+// eps^2, tol, then tp_offset[S+1], m, rc, mx, mv (DH), and the six TP arrays
+// (DH velocities), all little-endian raw.
+static const char MINT_CKPT_MAGIC[8] = {'M', 'I', 'N', 'T', 'C', 'K', 'P', '1'};
+
+int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_save_checkpoint: nothing uploaded");
+  if (!path) return c->fail(MINT_EINVAL, "mint_save_checkpoint: null path");
+  hipSetDevice(c->cfg.device);
+  int rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return c->fail(MINT_EIO, std::string("mint_save_checkpoint: cannot open ") + path);
+  const int32_t hdr_i[3] = {c->S, c->NM, 1};
+  const int64_t hdr_l[2] = {c->Ntp, step_index};
+  const double hdr_d[5] = {c->cfg.dt, c->cfg.central_mass, c->cfg.n_hill, c->cfg.softening2, c->cfg.bs_tol};
+  bool ok = std::fwrite(MINT_CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr_i, sizeof(hdr_i), 1, f) == 1 &&
+            std::fwrite(hdr_l, sizeof(hdr_l), 1, f) == 1 && std::fwrite(hdr_d, sizeof(hdr_d), 1, f) == 1;
+  ok = ok && std::fwrite(c->tp_offset.data(), sizeof(long long), c->tp_offset.size(), f) == c->tp_offset.size();
+  ok = ok && std::fwrite(c->m.data(), sizeof(double), c->m.size(), f) == c->m.size();
+  ok = ok && std::fwrite(c->rc.data(), sizeof(double), c->rc.size(), f) == c->rc.size();
+  const size_t nb = (size_t)c->S * c->NM * 3;
+  std::vector<double> buf(std::max<size_t>(nb, (size_t)c->Ntp));
+  for (double *d : {c->d_mx, c->d_mv}) {
+    if (ok && (rc = c->hip(hipMemcpy(buf.data(), d, nb * sizeof(double), hipMemcpyDeviceToHost), "D2H"))) {
+      std::fclose(f);
+      return rc;
+    }
+    ok = ok && std::fwrite(buf.data(), sizeof(double), nb, f) == nb;
+  }
+  for (int k = 0; k < 6 && ok; ++k) {
+    if (c->Ntp > 0 && (rc = c->hip(hipMemcpy(buf.data(), c->d_tp[k], (size_t)c->Ntp * sizeof(double),
+                                             hipMemcpyDeviceToHost), "D2H"))) {
+      std::fclose(f);
+      return rc;
+    }
+    ok = ok && std::fwrite(buf.data(), sizeof(double), (size_t)c->Ntp, f) == (size_t)c->Ntp;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? MINT_OK : c->fail(MINT_EIO, "mint_save_checkpoint: write failed");
+}
+
+int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  if (!path) return c->fail(MINT_EINVAL, "mint_load_checkpoint: null path");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return c->fail(MINT_EIO, std::string("mint_load_checkpoint: cannot open ") + path);
+  char magic[8];
+  int32_t hdr_i[3];
+  int64_t hdr_l[2];
+  double hdr_d[5];
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, MINT_CKPT_MAGIC, 8) == 0 &&
+            std::fread(hdr_i, sizeof(hdr_i), 1, f) == 1 && std::fread(hdr_l, sizeof(hdr_l), 1, f) == 1 &&
+            std::fread(hdr_d, sizeof(hdr_d), 1, f) == 1;
+  const int S = ok ? hdr_i[0] : 0, NM = ok ? hdr_i[1] : 0;
+  const int64_t N = ok ? hdr_l[0] : 0;
+  ok = ok && S > 0 && NM >= 1 && NM <= MAX_NM && N >= 0 && N < 0x7fffffff && hdr_i[2] == 1;
+  if (!ok) {
+    std::fclose(f);
+    return c->fail(MINT_EIO, "mint_load_checkpoint: not a mint checkpoint or truncated header");
+  }
+  std::vector<int64_t> off((size_t)S + 1);
+  std::vector<double> m((size_t)S * NM), rcv((size_t)S * NM), mx((size_t)S * NM * 3), mv((size_t)S * NM * 3);
+  std::vector<double> t[6];
+  for (auto &a : t) a.resize((size_t)N);
+  ok = std::fread(off.data(), sizeof(int64_t), off.size(), f) == off.size() &&
+       std::fread(m.data(), sizeof(double), m.size(), f) == m.size() &&
+       std::fread(rcv.data(), sizeof(double), rcv.size(), f) == rcv.size() &&
+       std::fread(mx.data(), sizeof(double), mx.size(), f) == mx.size() &&
+       std::fread(mv.data(), sizeof(double), mv.size(), f) == mv.size();
+  for (int k = 0; k < 6 && ok; ++k) ok = std::fread(t[k].data(), sizeof(double), (size_t)N, f) == (size_t)N;
+  std::fclose(f);
+  if (!ok || off[0] != 0 || off[S] != N) return c->fail(MINT_EIO, "mint_load_checkpoint: truncated or inconsistent file");
+  c->cfg.dt = hdr_d[0];
+  c->cfg.central_mass = hdr_d[1];
+  c->cfg.n_hill = hdr_d[2];
+  c->cfg.softening2 = hdr_d[3];
+  c->cfg.bs_tol = hdr_d[4];
+  // upload with the heliocentric conversion undone: pass DH velocities shifted
+  // back is lossy, so upload the geometry and overwrite the state raw
+  std::vector<double> zeros_v(mv.size(), 0.0);
+  mint_state st{S, NM, off.data(), m.data(), mx.data(), zeros_v.data(), t[0].data(), t[1].data(), t[2].data(),
+                t[3].data(), t[4].data(), t[5].data(), rcv.data()};
+  int rc = mint_upload(ctx, &st);
+  if (rc) return rc;
+  hipSetDevice(c->cfg.device);
+  if ((rc = c->hip(hipMemcpy(c->d_mv, mv.data(), mv.size() * sizeof(double), hipMemcpyHostToDevice), "H2D"))) return rc;
+  for (int k = 3; k < 6; ++k)
+    if (N > 0 && (rc = c->hip(hipMemcpy(c->d_tp[k], t[k].data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice),
+                              "H2D")))
+      return rc;
+  if (step_index) *step_index = hdr_l[1];
+  return MINT_OK;
+}
+
 // E = sum_j m_j |v_j|^2/2 + |sum_j m_j v_j|^2/(2M) - sum_j M m_j/|x_j|
 //     - sum_{j<k} m_j m_k/|x_j - x_k|   (democratic-heliocentric Hamiltonian)
 int mint_energy(void *ctx, double *E) {

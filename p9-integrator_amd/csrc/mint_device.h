@@ -85,18 +85,32 @@ __device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double
     z *= 0.25;
     ++n;
   }
-  double t2 = 1.1470745597729725e-11;
-  t2 = fma(t2, z, -2.08767569878681e-09);
-  t2 = fma(t2, z, 2.755731922398589e-07);
-  t2 = fma(t2, z, -2.48015873015873e-05);
-  t2 = fma(t2, z, 0.001388888888888889);
+  // Series length by |z| band: the first omitted term is below 3e-19 of the
+  // sum in every band (|z| <= 1e-5: degree 2, <= 1e-3: degree 3, else 6).
+  const double az = fabs(z);
+  double t2, t3;
+  if (az <= 1e-5) {
+    t2 = 0.001388888888888889;
+    t3 = 0.0001984126984126984;
+  } else if (az <= 1e-3) {
+    t2 = -2.48015873015873e-05;
+    t3 = -2.7557319223985893e-06;
+    t2 = fma(t2, z, 0.001388888888888889);
+    t3 = fma(t3, z, 0.0001984126984126984);
+  } else {
+    t2 = 1.1470745597729725e-11;
+    t2 = fma(t2, z, -2.08767569878681e-09);
+    t2 = fma(t2, z, 2.755731922398589e-07);
+    t2 = fma(t2, z, -2.48015873015873e-05);
+    t2 = fma(t2, z, 0.001388888888888889);
+    t3 = 7.647163731819816e-13;
+    t3 = fma(t3, z, -1.6059043836821613e-10);
+    t3 = fma(t3, z, 2.505210838544172e-08);
+    t3 = fma(t3, z, -2.7557319223985893e-06);
+    t3 = fma(t3, z, 0.0001984126984126984);
+  }
   t2 = fma(t2, z, -0.041666666666666664);
   t2 = fma(t2, z, 0.5);
-  double t3 = 7.647163731819816e-13;
-  t3 = fma(t3, z, -1.6059043836821613e-10);
-  t3 = fma(t3, z, 2.505210838544172e-08);
-  t3 = fma(t3, z, -2.7557319223985893e-06);
-  t3 = fma(t3, z, 0.0001984126984126984);
   t3 = fma(t3, z, -0.008333333333333333);
   t3 = fma(t3, z, 0.16666666666666666);
   c2 = t2;
@@ -249,13 +263,14 @@ __device__ __forceinline__ int kepler_drift_series(double &x, double &y, double 
 
 // Changeover-weighted pair acceleration factor (src/sim.cuh:166-186,327-382):
 // y = (r - 0.1 r_c)/(0.9 r_c); K = 0 (y<=0), y^2/(2y^2-2y+1) (0<y<1), 1 (y>=1).
-// The kick weight is K * m / ((r^2 + eps^2) r), folded into one division.
-__device__ __forceinline__ double kick_weight(double r2, double eps2, double m, double rclo,
-                                              double rcwinv) {
+// The kick weight is K * m / ((r^2 + eps^2) r) = num / den.
+__device__ __forceinline__ void kick_nd(double r2, double eps2, double m, double rclo,
+                                        double rcwinv, double &num, double &den) {
   const double r = sqrt(r2);
   const double D = (r2 + eps2) * r;
   const double yv = (r - rclo) * rcwinv;
-  double num = m, den = D;
+  num = m;
+  den = D;
   if (yv <= 0.0) {
     num = 0.0;
   } else if (yv < 1.0) {
@@ -263,7 +278,24 @@ __device__ __forceinline__ double kick_weight(double r2, double eps2, double m, 
     num = m * y2;
     den = D * fma(2.0, y2, fma(-2.0, yv, 1.0));
   }
-  return num / den;
+}
+
+// w_i = num_i / den_i for n terms with one division (batched reciprocal):
+// P_i = den_0 ... den_i; inv = 1/P_{n-1}; for i = n-1..1: w_i = num_i (inv
+// P_{i-1}), inv *= den_i; w_0 = num_0 inv.
+template <int N>
+__device__ __forceinline__ void batched_div(const double *num, const double *den, double *w) {
+  double P[N];
+  P[0] = den[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) P[i] = P[i - 1] * den[i];
+  double inv = 1.0 / P[N - 1];
+#pragma unroll
+  for (int i = N - 1; i >= 1; --i) {
+    w[i] = num[i] * (inv * P[i - 1]);
+    inv = inv * den[i];
+  }
+  w[0] = num[0] * inv;
 }
 
 // Close-encounter weight (1 - K) m / ((r^2 + eps^2) r) = (1-y)^2/(2y^2-2y+1) ...
@@ -290,20 +322,25 @@ __device__ __forceinline__ void accel_tp(double x, double y, double z,
                                          const double *__restrict__ X,
                                          const double *__restrict__ prm, double eps2,
                                          double &ax, double &ay, double &az) {
+  double dx[NM], dy[NM], dz[NM], num[NM], den[NM], w[NM];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    dx[j] = X[3 * j + 0] - x;
+    dy[j] = X[3 * j + 1] - y;
+    dz[j] = X[3 * j + 2] - z;
+    const double r2 = dot3(dx[j], dy[j], dz[j], dx[j], dy[j], dz[j]);
+    kick_nd(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
+            den[j]);
+  }
+  batched_div<NM>(num, den, w);
   ax = 0.0;
   ay = 0.0;
   az = 0.0;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
-    const double dx = X[3 * j + 0] - x;
-    const double dy = X[3 * j + 1] - y;
-    const double dz = X[3 * j + 2] - z;
-    const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double w = kick_weight(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j],
-                                 prm[P_RCWINV(NM) + j]);
-    ax = fma(w, dx, ax);
-    ay = fma(w, dy, ay);
-    az = fma(w, dz, az);
+    ax = fma(w[j], dx[j], ax);
+    ay = fma(w[j], dy[j], ay);
+    az = fma(w[j], dz[j], az);
   }
 }
 
@@ -487,18 +524,28 @@ __device__ __forceinline__ void accel_massive(const double *X, const double *__r
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     double ax = 0.0, ay = 0.0, az = 0.0;
+    if (NM > 1) {
+      constexpr int NP = NM > 1 ? NM - 1 : 1;
+      double dx[NP], dy[NP], dz[NP], num[NP], den[NP], w[NP];
+      int q = 0;
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      if (k == j) continue;
-      const double dx = X[3 * k + 0] - X[3 * j + 0];
-      const double dy = X[3 * k + 1] - X[3 * j + 1];
-      const double dz = X[3 * k + 2] - X[3 * j + 2];
-      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double w = kick_weight(r2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
-                                   prm[P_PRCWINV(NM) + j * NM + k]);
-      ax = fma(w, dx, ax);
-      ay = fma(w, dy, ay);
-      az = fma(w, dz, az);
+      for (int k = 0; k < NM; ++k) {
+        if (k == j) continue;
+        dx[q] = X[3 * k + 0] - X[3 * j + 0];
+        dy[q] = X[3 * k + 1] - X[3 * j + 1];
+        dz[q] = X[3 * k + 2] - X[3 * j + 2];
+        const double r2 = dot3(dx[q], dy[q], dz[q], dx[q], dy[q], dz[q]);
+        kick_nd(r2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
+                prm[P_PRCWINV(NM) + j * NM + k], num[q], den[q]);
+        ++q;
+      }
+      batched_div<NP>(num, den, w);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        ax = fma(w[p], dx[p], ax);
+        ay = fma(w[p], dy[p], ay);
+        az = fma(w[p], dz[p], az);
+      }
     }
     A[3 * j + 0] = ax;
     A[3 * j + 1] = ay;

@@ -121,6 +121,8 @@ _SIGS = {
     "mint_nn_get_weights": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "mint_nn_set_weights": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "mint_set_score_threshold": (C.c_int, [C.c_void_p, C.c_float]),
+    "mint_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "mint_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
 }
 
 
@@ -278,6 +280,21 @@ class Mint:
             st.tx[:, d] = mm.t[d]
             st.tv[:, d] = mm.t[3 + d]
         return st
+
+    def save_checkpoint(self, path, step_index=0):
+        self._chk(self.L.mint_save_checkpoint(self.h, path.encode(), int(step_index)), "mint_save_checkpoint")
+
+    def load_checkpoint(self, path) -> int:
+        k = C.c_int64()
+        self._chk(self.L.mint_load_checkpoint(self.h, path.encode(), C.byref(k)), "mint_load_checkpoint")
+        S = self.L  # shape comes from the file; read it back through a download
+        del S
+        hdr = np.fromfile(path, dtype=np.int32, count=5)[2:5]
+        n = int(np.fromfile(path, dtype=np.int64, count=1, offset=20)[0])
+        off = np.fromfile(path, dtype=np.int64, count=int(hdr[0]) + 1, offset=20 + 16 + 40)
+        self._shape = (int(hdr[0]), int(hdr[1]), off)
+        assert int(off[-1]) == n
+        return k.value
 
     def energy(self) -> np.ndarray:
         E = np.empty(self._shape[0])

@@ -111,3 +111,24 @@ def test_cli_runs_a_reference_schema_config(tmp_path):
     assert len(lines) == 3 * 8 and all(len(ln.split()) == 3 for ln in lines)
     el = (tmp_path / "elements_0.txt").read_text()
     assert el.startswith("Timestep: 40") and el.count("Experiment:") == 3
+
+
+def test_checkpoint_resume_is_bitwise(tmp_path):
+    st = W.p9_sweep(systems=np.arange(5), tp_scale=1 / 4)
+    path = str(tmp_path / "run.ckpt")
+    kw = dict(dt=0.5, form=mint.FORM_BATCHED, chunk_steps=32, block_size=128, tps_per_lane=1)
+    with mint.Mint(**kw) as g:
+        g.upload(st)
+        g.step(100)
+        ref = g.download()
+    with mint.Mint(**kw) as g:
+        g.upload(st)
+        g.step(60)
+        g.save_checkpoint(path, 60)
+    with mint.Mint(**kw) as g:
+        k = g.load_checkpoint(path)
+        assert k == 60
+        g.step(40)
+        out = g.download()
+    assert np.array_equal(out.tx, ref.tx) and np.array_equal(out.tv, ref.tv)
+    assert np.array_equal(out.mx, ref.mx) and np.array_equal(out.mv, ref.mv)

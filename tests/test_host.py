@@ -115,3 +115,11 @@ def test_cli_usage_and_bad_json(tmp_path):
     assert r.returncode == 1 and "massive" in r.stderr
     r = _cli(["-c", "@CFG", "-t"], tmp_path, json.dumps({"bodies": [body]}))
     assert r.returncode == 2
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_cli_rejects_missing_checkpoint_argument(tmp_path):
+    body = dict(name="b", mass=1e-6, semi_major_axis=1.0, eccentricity=0.0, inclination=0.0,
+                longitude_of_ascending_node=0.0, argument_of_perihelion=0.0, mean_anomaly=0.0)
+    r = _cli(["-c", "@CFG", "-r"], tmp_path, json.dumps({"bodies": [body]}))
+    assert r.returncode == 2 and "usage" in r.stderr
