@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MINT_ABI_VERSION 1
+#define MINT_ABI_VERSION 2
 
 /* error codes */
 #define MINT_OK 0
@@ -39,6 +39,7 @@ extern "C" {
 #define MINT_ENOMEM (-3)   /* device or host allocation failed */
 #define MINT_ESTATE (-4)   /* call out of order (e.g. step before upload) */
 #define MINT_EIO (-5)      /* file or parse error (config, weights, cache) */
+#define MINT_ELIMIT (-6)   /* a many-body capacity limit was exceeded (see MINT_ST_LIMIT) */
 
 /* Drift forms (how the TP population is scheduled on the GPU). */
 #define MINT_FORM_PLAIN 0       /* one launch per step, a few systems per launch, in-place BS */
@@ -55,6 +56,10 @@ extern "C" {
 #define MINT_ST_NEWTON 2u           /* a Kepler solve hit the iteration cap */
 #define MINT_ST_BS_DEPTH 4u         /* a Bulirsch-Stoer step hit its subdivision cap */
 #define MINT_ST_MASSIVE_ENC 8u      /* massive bodies came within their changeover radius */
+#define MINT_ST_OVERLAP 16u         /* two massive bodies closer than the sum of their physical
+                                       radii (recorded only: nothing merges, as in the reference) */
+#define MINT_ST_LIMIT 32u           /* many-body engine: more encounter pairs or a larger
+                                       encounter group than it holds; the state is not valid */
 
 /*
  * Integration parameters.  Replaces the compile-time constants of
@@ -78,6 +83,8 @@ typedef struct mint_config {
   int32_t device;        /* HIP device ordinal; default 0 (the -d flag, src/simutils.cuh:170-173) */
   int32_t collect_stats; /* bit 0: count work into mint_stats; bit 1: per-kernel HIP-event
                             timing into mint_timing (launches without the graph); default 0 */
+  int32_t many_body;     /* 1: batches without TPs use the many-body engine at any n_massive
+                            (same results; default 0 = only above 8 massive bodies) */
 } mint_config;
 
 /*
@@ -87,6 +94,9 @@ typedef struct mint_config {
  *   tp_offset[S+1]   prefix sums of TP counts; TPs of system s are
  *                    tp_offset[s] .. tp_offset[s+1]-1 in the SoA arrays
  *   m[S*Nm]          massive-body masses (> 1e-12, src/mercurius.cu:9)
+ *                    Nm = 1..8 with test particles; any Nm >= 1 without them
+ *                    (systems of more than 8 massive bodies run on the
+ *                    many-body engine: O(Nm^2) kicks, encounter groups)
  *   mx, mv[S*Nm*3]   heliocentric positions / velocities of massive bodies
  *   tx..tvz[Ntp]     heliocentric TP positions / velocities, SoA
  *   rc[S*Nm]         changeover radii; NULL on upload = computed from n_hill
@@ -113,6 +123,9 @@ typedef struct mint_stats {
   int64_t prebucketed;     /* batched: TPs routed to the queue by the scorer */
   int64_t idle_lane_steps; /* batched: lane-steps left idle in the main kernel */
   int64_t enc_tp_steps;    /* batched: TP steps taken by the encounter-queue kernel */
+  int64_t massive_groups;  /* massive encounter groups drifted by Bulirsch-Stoer (group-steps) */
+  int64_t massive_group_bodies; /* bodies in those groups (body-steps) */
+  int64_t overlaps;        /* massive pairs closer than the sum of their radii (pair-steps) */
 } mint_stats;
 
 /* Per-kernel timings from HIP events on the context stream (ms, launches). */
@@ -121,6 +134,10 @@ typedef struct mint_timing {
   double ms_tp_main;    int64_t n_tp_main;
   double ms_encounter;  int64_t n_encounter;
   double ms_total;
+  /* many-body engine: encounter pairs and groups, drift, jump + kicks */
+  double ms_mb_open;    int64_t n_mb_open;
+  double ms_mb_drift;   int64_t n_mb_drift;
+  double ms_mb_close;   int64_t n_mb_close;
 } mint_timing;
 
 int32_t mint_abi_version(void);
@@ -159,8 +176,17 @@ int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index);
  * tolerance) by the checkpoint's; *step_index receives the saved index. */
 int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index);
 
+/* Physical radii of the massive bodies, radius[S*Nm] (0 = none), after
+ * mint_upload.  They are recorded and checked, never used to merge: pairs
+ * closer than the sum of their radii at the encounter check are counted
+ * (mint_stats.overlaps) and flagged (MINT_ST_OVERLAP).  The reference keeps
+ * no physical radii (its Body, src/simutils.cuh:19-29, has none). */
+int mint_set_radii(void *ctx, const double *radius);
+
 /* Total energy of each system's massive bodies (democratic-heliocentric
- * Hamiltonian; the reference has no energy diagnostic). E[S]. */
+ * Hamiltonian; the reference has no energy diagnostic). E[S].  The pair
+ * potential is the one of the softened kick force m_j m_k d/((r^2+eps^2) r):
+ * -m_j m_k atan(eps/r)/eps (Newtonian when softening2 = 0). */
 int mint_energy(void *ctx, double *E);
 
 /* OR of status bits seen per system since upload. status[S]. */

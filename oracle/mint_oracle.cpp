@@ -7,7 +7,9 @@
 // of at2005/p9-integrator) with the documented semantic decisions applied
 // (universal-variable Kepler drift, r_c = n_hill Hill radii fixed at set-up,
 // tolerance Bulirsch-Stoer on flagged bodies only, moving perturbers, one
-// changeover per pair).  It is written independently of the GPU code but
+// changeover per pair, massive encounter groups integrated together, sums
+// over massive bodies in fixed 16-body chunks).  It is written independently
+// of the GPU code but
 // follows the same stated operation order (every fma is explicit, the file is
 // built with -ffp-contract=off), so the two agree to the last bit.
 //
@@ -48,6 +50,7 @@ constexpr int NEWTON_MAXIT = 40;
 
 struct Stats {
   int64_t tp_steps = 0, kepler = 0, newton = 0, bs_steps = 0, bs_rhs = 0;
+  int64_t mgroups = 0, mgroup_bodies = 0, overlaps = 0;
 };
 
 // --- Stumpff functions (Danby 1988 §6.9), replacing the Kepler-equation
@@ -172,6 +175,20 @@ struct Rc {
 };
 Rc make_rc(double rc) { return {0.1 * rc, 1.0 / (0.9 * rc), rc * rc}; }
 
+// Changeover of a massive pair: r_c = max(r_c,j, r_c,k).  Each field of
+// make_rc is a monotone, correctly rounded function of r_c, so the pair values
+// are the max/min of the two bodies' fields exactly (0.1 r_c and r_c^2
+// increase with r_c, 1/(0.9 r_c) decreases).
+Rc pair_rc(const Rc &a, const Rc &b) {
+  return {a.lo > b.lo ? a.lo : b.lo, a.winv < b.winv ? a.winv : b.winv, a.r2 > b.r2 ? a.r2 : b.r2};
+}
+
+// Sums over massive bodies run in chunks of CH bodies (index order): each
+// chunk's sum starts from zero, and the chunk sums are added in chunk order.
+// Pair weights are divided in groups of DG bodies with one division each.
+constexpr int CH = 16;
+constexpr int DG = 8;
+
 // K m / ((r^2+eps^2) r) as a numerator / denominator pair
 void kick_nd(double r2, double eps2, double m, const Rc &c, double &num, double &den) {
   const double r = std::sqrt(r2);
@@ -196,7 +213,7 @@ double kick_w(double r2, double eps2, double m, const Rc &c) {
 
 // num[i] / den[i] for all i with one division: prefix products P, inv = 1/P_last,
 // walk back w_i = num_i (inv P_{i-1}), inv *= den_i, w_0 = num_0 inv
-constexpr int MAXB = 8;  // massive bodies per system
+constexpr int MAXB = 8;  // massive bodies per system with test particles; division group size
 
 void batched_div(int n, const double *num, const double *den, double *w) {
   double P[MAXB];
@@ -224,8 +241,8 @@ double close_w(double r2, double eps2, double m, const Rc &c) {
 struct Sys {
   int nm;
   std::vector<double> m;
-  std::vector<Rc> rc_tp;   // [j]
-  std::vector<Rc> rc_pair; // [j*nm+k], rc = max(rc_j, rc_k)
+  std::vector<Rc> rc_tp;   // [j]; massive pairs use pair_rc(rc_tp[j], rc_tp[k])
+  std::vector<double> rad; // physical radii (empty: not recorded)
   std::vector<Body> mb;    // massive, heliocentric x, DH v
   std::vector<Body> tp;
 };
@@ -233,7 +250,7 @@ struct Sys {
 // kick acceleration of a TP (body_interaction_kick, src/sim.cuh:327-382)
 V3 accel_tp(const V3 &x, const std::vector<V3> &X, const Sys &s, double eps2) {
   V3 d[MAXB];
-  double num[MAXB], den[MAXB], w[MAXB];
+  double num[MAXB] = {}, den[MAXB] = {}, w[MAXB] = {};
   for (int j = 0; j < s.nm; ++j) {
     d[j] = sub(X[j], x);
     kick_nd(dot(d[j], d[j]), eps2, s.m[j], s.rc_tp[j], num[j], den[j]);
@@ -248,41 +265,70 @@ V3 accel_tp(const V3 &x, const std::vector<V3> &X, const Sys &s, double eps2) {
   return a;
 }
 
-std::vector<V3> accel_massive(const std::vector<V3> &X, const Sys &s, double eps2) {
-  std::vector<V3> A(s.nm);
-  for (int j = 0; j < s.nm; ++j) {
-    V3 a{0, 0, 0};
-    if (s.nm > 1) {
-      V3 d[MAXB];
-      double num[MAXB] = {}, den[MAXB] = {}, w[MAXB] = {};
-      int q = 0;
-      for (int k = 0; k < s.nm; ++k) {
-        if (k == j) continue;
-        d[q] = sub(X[k], X[j]);
-        kick_nd(dot(d[q], d[q]), eps2, s.m[k], s.rc_pair[j * s.nm + k], num[q], den[q]);
-        ++q;
+// Kick acceleration of massive body i from the others (body_interaction_kick,
+// src/sim.cuh:327-382, over massive partners, self excluded by index).  In a
+// division group the body itself is an inert slot (numerator 0, denominator 1,
+// separation 0), which leaves every other weight and the sum unchanged.
+V3 accel_one(int i, const std::vector<V3> &X, const Sys &s, double eps2) {
+  const int nm = s.nm;
+  V3 a{0, 0, 0};
+  for (int c0 = 0; c0 < nm; c0 += CH) {
+    const int c1 = std::min(c0 + CH, nm);
+    V3 p{0, 0, 0};
+    for (int g0 = c0; g0 < c1; g0 += DG) {
+      const int n = std::min(g0 + DG, c1) - g0;
+      V3 d[DG];
+      double num[DG] = {}, den[DG] = {}, w[DG] = {};
+      for (int q = 0; q < n; ++q) {
+        const int k = g0 + q;
+        if (k == i) {
+          d[q] = {0, 0, 0};
+          num[q] = 0.0;
+          den[q] = 1.0;
+        } else {
+          d[q] = sub(X[k], X[i]);
+          kick_nd(dot(d[q], d[q]), eps2, s.m[k], pair_rc(s.rc_tp[i], s.rc_tp[k]), num[q], den[q]);
+        }
       }
-      batched_div(q, num, den, w);
-      for (int p = 0; p < q; ++p) {
-        a.x = std::fma(w[p], d[p].x, a.x);
-        a.y = std::fma(w[p], d[p].y, a.y);
-        a.z = std::fma(w[p], d[p].z, a.z);
+      batched_div(n, num, den, w);
+      for (int q = 0; q < n; ++q) {
+        p.x = std::fma(w[q], d[q].x, p.x);
+        p.y = std::fma(w[q], d[q].y, p.y);
+        p.z = std::fma(w[q], d[q].z, p.z);
       }
     }
-    A[j] = a;
+    if (c0 == 0) {
+      a = p;
+    } else {
+      a = {a.x + p.x, a.y + p.y, a.z + p.z};
+    }
   }
+  return a;
+}
+
+std::vector<V3> accel_massive(const std::vector<V3> &X, const Sys &s, double eps2) {
+  std::vector<V3> A(s.nm);
+  for (int j = 0; j < s.nm; ++j) A[j] = accel_one(j, X, s, eps2);
   return A;
 }
 
-// Q = sum m_j v_j / M (main_body_kinetic, src/sim.cuh:384-406)
+// Q = sum m_j v_j / M (main_body_kinetic, src/sim.cuh:384-406), chunked sum
 V3 momentum_q(const Sys &s, double M) {
-  double px = 0, py = 0, pz = 0;
-  for (int j = 0; j < s.nm; ++j) {
-    px = std::fma(s.m[j], s.mb[j].v.x, px);
-    py = std::fma(s.m[j], s.mb[j].v.y, py);
-    pz = std::fma(s.m[j], s.mb[j].v.z, pz);
+  V3 q{0, 0, 0};
+  for (int c0 = 0; c0 < s.nm; c0 += CH) {
+    V3 p{0, 0, 0};
+    for (int j = c0; j < std::min(c0 + CH, s.nm); ++j) {
+      p.x = std::fma(s.m[j], s.mb[j].v.x, p.x);
+      p.y = std::fma(s.m[j], s.mb[j].v.y, p.y);
+      p.z = std::fma(s.m[j], s.mb[j].v.z, p.z);
+    }
+    if (c0 == 0) {
+      q = p;
+    } else {
+      q = {q.x + p.x, q.y + p.y, q.z + p.z};
+    }
   }
-  return {px / M, py / M, pz / M};
+  return {q.x / M, q.y / M, q.z / M};
 }
 
 // --- close-encounter RHS and Bulirsch-Stoer (src/sim.cuh:410-551) -------------
@@ -404,47 +450,94 @@ uint32_t bs_tp(Body &b, const std::vector<Body> &pert0, const Sys &s, const Para
   return st;
 }
 
-uint32_t bs_massive(Sys &s, const Params &p) {
-  const int nm = s.nm;
-  std::vector<double> y(6 * nm);
-  for (int j = 0; j < nm; ++j) {
-    const Body &b = s.mb[j];
-    double *q = y.data() + 6 * j;
+// BS drift of one massive encounter group (members in index order): each
+// member feels the Sun and the (1-K)-weighted pulls of the other members.
+uint32_t bs_group(Sys &s, const std::vector<int> &mem, const Params &p, int64_t &nrhs) {
+  const int g = (int)mem.size();
+  std::vector<double> y(6 * g);
+  for (int a = 0; a < g; ++a) {
+    const Body &b = s.mb[mem[a]];
+    double *q = y.data() + 6 * a;
     q[0] = b.x.x; q[1] = b.x.y; q[2] = b.x.z; q[3] = b.v.x; q[4] = b.v.y; q[5] = b.v.z;
   }
-  const std::vector<int> grp = groups_for(nm);
+  const std::vector<int> grp = groups_for(g);
   auto rhs = [&](double, const double *yy, double *f) {
-    for (int j = 0; j < nm; ++j) {
-      const double *q = yy + 6 * j;
-      f[6 * j + 0] = q[3];
-      f[6 * j + 1] = q[4];
-      f[6 * j + 2] = q[5];
+    for (int a = 0; a < g; ++a) {
+      const int j = mem[a];
+      const double *q = yy + 6 * a;
+      f[6 * a + 0] = q[3];
+      f[6 * a + 1] = q[4];
+      f[6 * a + 2] = q[5];
       const V3 x{q[0], q[1], q[2]};
       const double r2 = dot(x, x);
       const double r = std::sqrt(r2);
       const double w0 = p.mu / (r2 * r);
-      V3 a{-(w0 * x.x), -(w0 * x.y), -(w0 * x.z)};
-      for (int k = 0; k < nm; ++k) {
-        if (k == j) continue;
-        const V3 d{yy[6 * k] - x.x, yy[6 * k + 1] - x.y, yy[6 * k + 2] - x.z};
-        const double w = close_w(dot(d, d), p.eps2, s.m[k], s.rc_pair[j * nm + k]);
-        a.x = std::fma(w, d.x, a.x);
-        a.y = std::fma(w, d.y, a.y);
-        a.z = std::fma(w, d.z, a.z);
+      V3 acc{-(w0 * x.x), -(w0 * x.y), -(w0 * x.z)};
+      for (int b = 0; b < g; ++b) {
+        if (b == a) continue;
+        const int k = mem[b];
+        const V3 d{yy[6 * b] - x.x, yy[6 * b + 1] - x.y, yy[6 * b + 2] - x.z};
+        const double w = close_w(dot(d, d), p.eps2, s.m[k], pair_rc(s.rc_tp[j], s.rc_tp[k]));
+        acc.x = std::fma(w, d.x, acc.x);
+        acc.y = std::fma(w, d.y, acc.y);
+        acc.z = std::fma(w, d.z, acc.z);
       }
-      f[6 * j + 3] = a.x;
-      f[6 * j + 4] = a.y;
-      f[6 * j + 5] = a.z;
+      f[6 * a + 3] = acc.x;
+      f[6 * a + 4] = acc.y;
+      f[6 * a + 5] = acc.z;
     }
   };
-  int64_t nr = 0;
-  const uint32_t st = bs_drift(y, p.h, {}, rhs, grp, 2 * nm, p.tol, nr);
-  for (int j = 0; j < nm; ++j) {
-    const double *q = y.data() + 6 * j;
-    s.mb[j].x = {q[0], q[1], q[2]};
-    s.mb[j].v = {q[3], q[4], q[5]};
+  const uint32_t st = bs_drift(y, p.h, {}, rhs, grp, 2 * g, p.tol, nrhs);
+  for (int a = 0; a < g; ++a) {
+    const double *q = y.data() + 6 * a;
+    s.mb[mem[a]].x = {q[0], q[1], q[2]};
+    s.mb[mem[a]].v = {q[3], q[4], q[5]};
   }
   return st;
+}
+
+// Massive encounter groups: the connected components of the graph whose edges
+// are the pairs inside their changeover radius (|x_k - x_j|^2 < r_c^2 of the
+// pair).  Groups are listed by their smallest member, members in index order;
+// bodies outside every edge are absent.  Also counts the pairs closer than the
+// sum of their physical radii.
+std::vector<std::vector<int>> encounter_groups(const Sys &s, int64_t &overlaps) {
+  const int nm = s.nm;
+  std::vector<int> parent(nm);
+  std::vector<char> inedge(nm, 0);
+  for (int j = 0; j < nm; ++j) parent[j] = j;
+  auto find = [&](int a) {
+    while (parent[a] != a) a = parent[a];
+    return a;
+  };
+  const bool radii = !s.rad.empty();
+  for (int j = 0; j < nm; ++j)
+    for (int k = j + 1; k < nm; ++k) {
+      const V3 d = sub(s.mb[k].x, s.mb[j].x);
+      const double r2 = dot(d, d);
+      if (radii) {
+        const double sr = s.rad[j] + s.rad[k];
+        if (r2 < sr * sr) ++overlaps;
+      }
+      if (r2 < pair_rc(s.rc_tp[j], s.rc_tp[k]).r2) {
+        inedge[j] = inedge[k] = 1;
+        const int a = find(j), b = find(k);
+        if (a < b) parent[b] = a;
+        if (b < a) parent[a] = b;
+      }
+    }
+  std::vector<std::vector<int>> groups;
+  std::vector<int> gid(nm, -1);
+  for (int j = 0; j < nm; ++j) {
+    if (!inedge[j]) continue;
+    const int r = find(j);
+    if (gid[r] < 0) {
+      gid[r] = (int)groups.size();
+      groups.emplace_back();
+    }
+    groups[gid[r]].push_back(j);
+  }
+  return groups;
 }
 
 // --- one Mercurius step of one system (src/sim.cuh:679-742) -------------------
@@ -498,21 +591,24 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt) {
     }
     stt.tp_steps++;
   }
-  bool menc = false;
-  for (int j = 0; j < nm; ++j)
-    for (int k = j + 1; k < nm; ++k) {
-      const V3 d = sub(s.mb[k].x, s.mb[j].x);
-      menc = menc || (dot(d, d) < s.rc_pair[j * nm + k].r2);
-    }
-  if (menc) {
+  int64_t ov = 0;
+  const std::vector<std::vector<int>> groups = encounter_groups(s, ov);
+  stt.overlaps += ov;
+  std::vector<char> ingroup(nm, 0);
+  int64_t nr = 0;
+  for (const std::vector<int> &g : groups) {
     st |= 8u;
-    st |= bs_massive(s, p);
-  } else {
-    for (Body &b : s.mb) {
-      bool ok;
-      kepler_drift(b, p.mu, p.h, 1, ok);
-      if (!ok) st |= 2u;
-    }
+    st |= bs_group(s, g, p, nr);
+    for (int j : g) ingroup[j] = 1;
+    stt.mgroups++;
+    stt.mgroup_bodies += (int64_t)g.size();
+  }
+  if (ov) st |= 16u;
+  for (int j = 0; j < nm; ++j) {
+    if (ingroup[j]) continue;
+    bool ok;
+    kepler_drift(s.mb[j], p.mu, p.h, 1, ok);
+    if (!ok) st |= 2u;
   }
   // 5. jump 1/2 with the momentum after the drift
   const V3 Q2 = momentum_q(s, M);
@@ -555,6 +651,7 @@ struct OrcParams {
 
 struct OrcStats {
   int64_t tp_steps, kepler_solves, newton_iters, bs_steps, bs_rhs_evals;
+  int64_t massive_groups, massive_group_bodies, overlaps;
 };
 
 int orc_stumpff(double z, double *c) {
@@ -610,8 +707,9 @@ int orc_bs_tp(double *y6, int nm, const double *m, const double *rc, const doubl
 // and back after the last (src/sim.cuh:555-587,677,748).  rc[S*NM] are the
 // changeover radii.  threads > 1 spreads systems over std::thread.
 int orc_run(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, const double *m, const double *rc,
-            double *mx, double *mv, double *tx, double *ty, double *tz, double *tvx, double *tvy, double *tvz,
-            int64_t nsteps, uint32_t *status, OrcStats *stats) {
+            const double *rad, double *mx, double *mv, double *tx, double *ty, double *tz, double *tvx,
+            double *tvy, double *tvz, int64_t nsteps, uint32_t *status, OrcStats *stats) {
+  if (NM > MAXB && tp_off[S] > 0) return -1;  // test particles need at most MAXB massive bodies
   const double M = op->central_mass;
   const Params p{op->dt, M, op->softening2, op->bs_tol, op->guess};
   const int nthr = op->threads > 1 ? op->threads : 1;
@@ -623,9 +721,7 @@ int orc_run(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, c
       s.m.assign(m + (size_t)si * NM, m + (size_t)(si + 1) * NM);
       const double *rcs = rc + (size_t)si * NM;
       for (int j = 0; j < NM; ++j) s.rc_tp.push_back(make_rc(rcs[j]));
-      s.rc_pair.resize((size_t)NM * NM);
-      for (int j = 0; j < NM; ++j)
-        for (int k = 0; k < NM; ++k) s.rc_pair[j * NM + k] = make_rc(rcs[j] > rcs[k] ? rcs[j] : rcs[k]);
+      if (rad) s.rad.assign(rad + (size_t)si * NM, rad + (size_t)(si + 1) * NM);
       // DH velocities: v - (sum m v)/(M + sum m)
       double px = 0, py = 0, pz = 0, msum = 0;
       for (int j = 0; j < NM; ++j) {
@@ -689,15 +785,24 @@ int orc_run(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, c
       stats->newton_iters += s.newton;
       stats->bs_steps += s.bs_steps;
       stats->bs_rhs_evals += s.bs_rhs;
+      stats->massive_groups += s.mgroups;
+      stats->massive_group_bodies += s.mgroup_bodies;
+      stats->overlaps += s.overlaps;
     }
   }
   return 0;
 }
 
 // Energy of the massive bodies from heliocentric (x, v):
-// E = sum m v_dh^2/2 + |sum m v_dh|^2/(2M) - sum M m/r - sum_{j<k} m_j m_k/r_jk,
-// with v_dh = v - (sum m v)/(M + sum m).
-void orc_energy(int32_t S, int32_t NM, const double *m, const double *mx, const double *mv, double M, double *E) {
+// E = sum m v_dh^2/2 + |sum m v_dh|^2/(2M) - sum M m/r + sum_{j<k} m_j m_k U(r_jk),
+// with v_dh = v - (sum m v)/(M + sum m) and U the pair potential whose force is
+// the kick's m_j m_k d/((r^2 + eps^2) r): U(r) = -atan(eps/r)/eps (-1/r when
+// eps = 0).
+double pair_potential(double r, double eps) { return eps > 0.0 ? -std::atan(eps / r) / eps : -1.0 / r; }
+
+void orc_energy(int32_t S, int32_t NM, const double *m, const double *mx, const double *mv, double M, double eps2,
+                double *E) {
+  const double eps = std::sqrt(eps2);
   for (int s = 0; s < S; ++s) {
     const double *mm = m + (size_t)s * NM;
     double px = 0, py = 0, pz = 0, msum = 0;
@@ -725,12 +830,49 @@ void orc_energy(int32_t S, int32_t NM, const double *m, const double *mx, const 
       pot -= M * mm[j] / std::sqrt(dot(x[j], x[j]));
       for (int k = j + 1; k < NM; ++k) {
         const V3 d = sub(x[k], x[j]);
-        pot -= mm[j] * mm[k] / std::sqrt(dot(d, d));
+        pot += mm[j] * mm[k] * pair_potential(std::sqrt(dot(d, d)), eps);
       }
     }
     const double p2 = std::fma(qx, qx, std::fma(qy, qy, qz * qz));
     E[s] = kin + p2 / (2.0 * M) + pot;
   }
+}
+
+// Kick accelerations of NM massive bodies (heliocentric x, body-major xyz).
+void orc_accel_massive(int32_t NM, const double *m, const double *rc, const double *x, double eps2, double *a) {
+  Sys s;
+  s.nm = NM;
+  s.m.assign(m, m + NM);
+  for (int j = 0; j < NM; ++j) s.rc_tp.push_back(make_rc(rc[j]));
+  std::vector<V3> X(NM);
+  for (int j = 0; j < NM; ++j) X[j] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+  for (int j = 0; j < NM; ++j) {
+    const V3 q = accel_one(j, X, s, eps2);
+    a[3 * j] = q.x;
+    a[3 * j + 1] = q.y;
+    a[3 * j + 2] = q.z;
+  }
+}
+
+// Encounter groups of NM massive bodies at positions x: gid[j] = group number
+// (groups numbered by smallest member) or -1.  Returns the number of groups;
+// *overlaps counts pairs closer than the sum of their radii (rad may be null).
+int orc_encounter_groups(int32_t NM, const double *rc, const double *rad, const double *x, int32_t *gid,
+                         int64_t *overlaps) {
+  Sys s;
+  s.nm = NM;
+  s.m.assign(NM, 1.0);
+  for (int j = 0; j < NM; ++j) s.rc_tp.push_back(make_rc(rc[j]));
+  if (rad) s.rad.assign(rad, rad + NM);
+  s.mb.resize(NM);
+  for (int j = 0; j < NM; ++j) s.mb[j].x = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+  int64_t ov = 0;
+  const std::vector<std::vector<int>> g = encounter_groups(s, ov);
+  for (int j = 0; j < NM; ++j) gid[j] = -1;
+  for (size_t q = 0; q < g.size(); ++q)
+    for (int j : g[q]) gid[j] = (int32_t)q;
+  if (overlaps) *overlaps = ov;
+  return (int)g.size();
 }
 
 }  // extern "C"

@@ -21,7 +21,8 @@ class OrcParams(C.Structure):
 
 
 class OrcStats(C.Structure):
-    _fields_ = [(n, C.c_int64) for n in ("tp_steps", "kepler_solves", "newton_iters", "bs_steps", "bs_rhs_evals")]
+    _fields_ = [(n, C.c_int64) for n in ("tp_steps", "kepler_solves", "newton_iters", "bs_steps", "bs_rhs_evals",
+                                         "massive_groups", "massive_group_bodies", "overlaps")]
 
 
 _L = None
@@ -45,9 +46,13 @@ def lib():
                                 C.c_double, C.POINTER(C.c_int64)]
         L.orc_bs_tp.restype = C.c_int
         L.orc_run.argtypes = [C.POINTER(OrcParams), C.c_int32, C.c_int32, C.POINTER(C.c_int64), _DP, _DP,
-                              _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int64, C.POINTER(C.c_uint32),
+                              _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int64, C.POINTER(C.c_uint32),
                               C.POINTER(OrcStats)]
-        L.orc_energy.argtypes = [C.c_int32, C.c_int32, _DP, _DP, _DP, C.c_double, _DP]
+        L.orc_run.restype = C.c_int
+        L.orc_energy.argtypes = [C.c_int32, C.c_int32, _DP, _DP, _DP, C.c_double, C.c_double, _DP]
+        L.orc_accel_massive.argtypes = [C.c_int32, _DP, _DP, _DP, C.c_double, _DP]
+        L.orc_encounter_groups.argtypes = [C.c_int32, _DP, _DP, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.orc_encounter_groups.restype = C.c_int
         _L = L
     return _L
 
@@ -98,9 +103,11 @@ def bs_tp(y6, m, rc, px, pv, h, mu=1.0, eps2=1e-8, tol=1e-12):
     return y, st, n.value
 
 
-def run(state, nsteps, dt, central_mass=1.0, softening2=1e-8, bs_tol=1e-12, guess=1, threads=1, n_hill=3.0):
+def run(state, nsteps, dt, central_mass=1.0, softening2=1e-8, bs_tol=1e-12, guess=1, threads=1, n_hill=3.0,
+        radii=None):
     """Integrate a copy of `state` (a mint.State-like object); returns
-    (new_state_arrays dict, status, stats)."""
+    (new_state_arrays dict, status, stats).  radii: optional physical radii
+    [S, NM] (overlaps are counted, nothing merges)."""
     S, NM = state.m.shape
     off = np.ascontiguousarray(state.tp_offset, np.int64)
     m = np.ascontiguousarray(state.m, np.float64)
@@ -113,16 +120,41 @@ def run(state, nsteps, dt, central_mass=1.0, softening2=1e-8, bs_tol=1e-12, gues
     status = np.zeros(S, np.uint32)
     stats = OrcStats()
     p = OrcParams(dt, central_mass, softening2, bs_tol, guess, threads)
-    lib().orc_run(C.byref(p), S, NM, off.ctypes.data_as(C.POINTER(C.c_int64)), _d(m), _d(rc), _d(mx), _d(mv),
-                  *[_d(a) for a in t], int(nsteps), status.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(stats))
+    rad = None if radii is None else np.ascontiguousarray(radii, np.float64).reshape(S, NM)
+    rc_ = lib().orc_run(C.byref(p), S, NM, off.ctypes.data_as(C.POINTER(C.c_int64)), _d(m), _d(rc),
+                        None if rad is None else _d(rad), _d(mx), _d(mv), *[_d(a) for a in t], int(nsteps),
+                        status.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(stats))
+    if rc_ != 0:
+        raise ValueError("oracle: test particles need at most 8 massive bodies")
     out = dict(mx=mx, mv=mv, tx=np.stack(t[:3], 1), tv=np.stack(t[3:], 1), rc=rc)
     return out, status, {n: getattr(stats, n) for n, _ in OrcStats._fields_}
 
 
-def energy(m, mx, mv, M=1.0):
+def energy(m, mx, mv, M=1.0, eps2=1e-8):
     m = np.ascontiguousarray(m, np.float64)
     S, NM = m.shape
     E = np.empty(S)
     lib().orc_energy(S, NM, _d(m), _d(np.ascontiguousarray(mx, np.float64)), _d(np.ascontiguousarray(mv, np.float64)),
-                     M, _d(E))
+                     M, eps2, _d(E))
     return E
+
+
+def accel_massive(m, rc, x, eps2=1e-8):
+    m = np.ascontiguousarray(m, np.float64).ravel()
+    rc = np.ascontiguousarray(rc, np.float64).ravel()
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    a = np.empty_like(x)
+    lib().orc_accel_massive(len(m), _d(m), _d(rc), _d(x), eps2, _d(a))
+    return a
+
+
+def encounter_groups(rc, x, radii=None):
+    """(gid[NM] with -1 for bodies in no group, number of groups, overlaps)."""
+    rc = np.ascontiguousarray(rc, np.float64).ravel()
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    rad = None if radii is None else np.ascontiguousarray(radii, np.float64).ravel()
+    gid = np.empty(len(rc), np.int32)
+    ov = C.c_int64()
+    n = lib().orc_encounter_groups(len(rc), _d(rc), None if rad is None else _d(rad), _d(x),
+                                   gid.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(ov))
+    return gid, n, ov.value

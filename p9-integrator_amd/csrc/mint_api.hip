@@ -66,6 +66,13 @@ void Ctx::free_device() {
   d_status = nullptr;
   d_stats = nullptr;
   d_nnw_kep = d_nnw_sc = nullptr;
+  if (mb_graph) hipGraphExecDestroy(mb_graph);
+  mb_graph = nullptr;
+  mb_graph_steps = 0;
+  d_bp = d_mxb = d_mvb = d_my = nullptr;
+  d_mflag = d_medge = d_gstart = d_gmem = nullptr;
+  d_mctl = nullptr;
+  many = false;
   rec_steps_alloc = 0;
   tiles_key = -1;
   uploaded = false;
@@ -323,8 +330,8 @@ int mint_upload(void *ctx, const mint_state *st) {
   if (!c) return MINT_EINVAL;
   if (!st) return c->fail(MINT_EINVAL, "mint_upload: null state");
   const int S = st->n_systems, NM = st->n_massive;
-  if (S <= 0 || NM < 1 || NM > MAX_NM)
-    return c->fail(MINT_EINVAL, "mint_upload: need n_systems > 0 and 1 <= n_massive <= 8");
+  if (S <= 0 || NM < 1 || NM > MB_MAXN)
+    return c->fail(MINT_EINVAL, "mint_upload: need n_systems > 0 and 1 <= n_massive <= 2048");
   if (!st->tp_offset || !st->m || !st->mx || !st->mv)
     return c->fail(MINT_EINVAL, "mint_upload: missing tp_offset/m/mx/mv");
   if (st->tp_offset[0] != 0) return c->fail(MINT_EINVAL, "mint_upload: tp_offset[0] must be 0");
@@ -335,6 +342,9 @@ int mint_upload(void *ctx, const mint_state *st) {
   if (N >= (int64_t)0x7fffffff) return c->fail(MINT_EINVAL, "mint_upload: at most 2^31-1 TPs");
   if (N > 0 && (!st->tx || !st->ty || !st->tz || !st->tvx || !st->tvy || !st->tvz))
     return c->fail(MINT_EINVAL, "mint_upload: missing TP arrays");
+  const bool many = NM > MAX_NM || (c->cfg.many_body && N == 0);
+  if (many && N > 0)
+    return c->fail(MINT_EINVAL, "mint_upload: test particles need at most 8 massive bodies per system");
   for (int i = 0; i < S * NM; ++i)
     if (!(st->m[i] > 1e-12))
       return c->fail(MINT_EINVAL, "mint_upload: massive masses must exceed 1e-12 (src/mercurius.cu:9)");
@@ -347,7 +357,9 @@ int mint_upload(void *ctx, const mint_state *st) {
   c->S = S;
   c->NM = NM;
   c->Ntp = N;
-  c->L = launchers_for(NM);
+  c->many = many;
+  c->L = many ? nullptr : launchers_for(NM);
+  c->rad.assign((size_t)S * NM, 0.0);
   c->tp_offset.assign(st->tp_offset, st->tp_offset + S + 1);
   c->m.assign(st->m, st->m + (size_t)S * NM);
   c->rc.resize((size_t)S * NM);
@@ -360,10 +372,10 @@ int mint_upload(void *ctx, const mint_state *st) {
     mint_hill_radii((int64_t)S * NM, c->cfg.central_mass, c->cfg.n_hill, st->m, st->mx, st->mv,
                     c->rc.data());
   }
-  // parameter blocks
-  const int PS = prm_size(NM);
+  // parameter blocks (the many-body engine keeps per-body parameters instead)
+  const int PS = many ? 0 : prm_size(NM);
   std::vector<double> prm((size_t)S * PS, 0.0);
-  for (int s = 0; s < S; ++s) {
+  for (int s = 0; s < S && !many; ++s) {
     double *p = prm.data() + (size_t)s * PS;
     const double *rcs = c->rc.data() + (size_t)s * NM;
     for (int j = 0; j < NM; ++j) {
@@ -401,9 +413,14 @@ int mint_upload(void *ctx, const mint_state *st) {
   int rc;
   const size_t mb = (size_t)S * NM * 3 * sizeof(double);
   if (!reuse) {
-  if ((rc = c->dalloc((void **)&c->d_prm, prm.size() * sizeof(double), "params"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mx, mb, "massive x"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mv, mb, "massive v"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_status, (size_t)S * sizeof(unsigned int), "status"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_stats, S_COUNT * sizeof(unsigned long long), "stats"))) return rc;
+  if (many && (rc = mb_alloc(c))) return rc;
+  }
+  if (!reuse && !many) {
+  if ((rc = c->dalloc((void **)&c->d_prm, prm.size() * sizeof(double), "params"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_xs0, mb, "chunk start positions"))) return rc;
   for (int d = 0; d < 6; ++d)
     if ((rc = c->dalloc((void **)&c->d_tp[d], (size_t)N * sizeof(double), "TP state"))) return rc;
@@ -412,13 +429,12 @@ int mint_upload(void *ctx, const mint_state *st) {
   if ((rc = c->dalloc((void **)&c->d_need, (size_t)S * sizeof(unsigned int), "cache flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_q, (size_t)std::max<int64_t>(N, 1) * sizeof(QEntry), "BS queue"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_qctl, 4 * sizeof(unsigned int), "queue counters"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_status, (size_t)S * sizeof(unsigned int), "status"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_stats, S_COUNT * sizeof(unsigned long long), "stats"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_kep, KM_NP * sizeof(float), "kepler mlp"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_sc, SC_NP * sizeof(float), "scorer"))) return rc;
   }
 
-  if ((rc = c->hip(hipMemcpy(c->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice), "H2D params"))) return rc;
+  if (!many && (rc = c->hip(hipMemcpy(c->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice), "H2D params"))) return rc;
+  if (many && (rc = mb_reset(c))) return rc;
   if ((rc = c->hip(hipMemcpy(c->d_mx, st->mx, mb, hipMemcpyHostToDevice), "H2D massive x"))) return rc;
   if ((rc = c->hip(hipMemcpy(c->d_mv, mv.data(), mb, hipMemcpyHostToDevice), "H2D massive v"))) return rc;
   const double *tpin[3] = {st->tx, st->ty, st->tz};
@@ -431,6 +447,12 @@ int mint_upload(void *ctx, const mint_state *st) {
   }
   if ((rc = c->hip(hipMemset(c->d_status, 0, (size_t)S * sizeof(unsigned int)), "status clear"))) return rc;
   if ((rc = c->hip(hipMemset(c->d_stats, 0, S_COUNT * sizeof(unsigned long long)), "stats clear"))) return rc;
+  if (many) {
+    if ((rc = c->hip(hipDeviceSynchronize(), "upload"))) return rc;
+    c->uploaded = true;
+    c->timing = mint_timing{};
+    return MINT_OK;
+  }
   if ((rc = c->hip(hipMemset(c->d_qctl, 0, 4 * sizeof(unsigned int)), "queue clear"))) return rc;
   // networks: keep any weights set earlier on this context
   if (!c->w_kep.empty())
@@ -491,9 +513,9 @@ int mint_download(void *ctx, mint_state *st) {
 }
 
 // ---------------------------------------------------------------------------
-// Checkpoint file: magic, version, S, NM, Ntp, step index, dt, M, n_hill,
-// eps^2, tol, then tp_offset[S+1], m, rc, mx, mv (DH), and the six TP arrays
-// (DH velocities), all little-endian raw.
+// Checkpoint file: magic, S, NM, format version (2), Ntp, step index, dt, M,
+// n_hill, eps^2, tol, then tp_offset[S+1], m, rc, physical radii, mx, mv (DH),
+// and the six TP arrays (DH velocities), all little-endian raw.
 static const char MINT_CKPT_MAGIC[8] = {'M', 'I', 'N', 'T', 'C', 'K', 'P', '1'};
 
 int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index) {
@@ -506,7 +528,7 @@ int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index) {
   if (rc) return rc;
   FILE *f = std::fopen(path, "wb");
   if (!f) return c->fail(MINT_EIO, std::string("mint_save_checkpoint: cannot open ") + path);
-  const int32_t hdr_i[3] = {c->S, c->NM, 1};
+  const int32_t hdr_i[3] = {c->S, c->NM, 2};
   const int64_t hdr_l[2] = {c->Ntp, step_index};
   const double hdr_d[5] = {c->cfg.dt, c->cfg.central_mass, c->cfg.n_hill, c->cfg.softening2, c->cfg.bs_tol};
   bool ok = std::fwrite(MINT_CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr_i, sizeof(hdr_i), 1, f) == 1 &&
@@ -514,6 +536,7 @@ int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index) {
   ok = ok && std::fwrite(c->tp_offset.data(), sizeof(long long), c->tp_offset.size(), f) == c->tp_offset.size();
   ok = ok && std::fwrite(c->m.data(), sizeof(double), c->m.size(), f) == c->m.size();
   ok = ok && std::fwrite(c->rc.data(), sizeof(double), c->rc.size(), f) == c->rc.size();
+  ok = ok && std::fwrite(c->rad.data(), sizeof(double), c->rad.size(), f) == c->rad.size();
   const size_t nb = (size_t)c->S * c->NM * 3;
   std::vector<double> buf(std::max<size_t>(nb, (size_t)c->Ntp));
   for (double *d : {c->d_mx, c->d_mv}) {
@@ -550,18 +573,20 @@ int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
             std::fread(hdr_d, sizeof(hdr_d), 1, f) == 1;
   const int S = ok ? hdr_i[0] : 0, NM = ok ? hdr_i[1] : 0;
   const int64_t N = ok ? hdr_l[0] : 0;
-  ok = ok && S > 0 && NM >= 1 && NM <= MAX_NM && N >= 0 && N < 0x7fffffff && hdr_i[2] == 1;
+  ok = ok && S > 0 && NM >= 1 && NM <= MB_MAXN && N >= 0 && N < 0x7fffffff && hdr_i[2] == 2;
   if (!ok) {
     std::fclose(f);
     return c->fail(MINT_EIO, "mint_load_checkpoint: not a mint checkpoint or truncated header");
   }
   std::vector<int64_t> off((size_t)S + 1);
-  std::vector<double> m((size_t)S * NM), rcv((size_t)S * NM), mx((size_t)S * NM * 3), mv((size_t)S * NM * 3);
+  std::vector<double> m((size_t)S * NM), rcv((size_t)S * NM), rad((size_t)S * NM), mx((size_t)S * NM * 3),
+      mv((size_t)S * NM * 3);
   std::vector<double> t[6];
   for (auto &a : t) a.resize((size_t)N);
   ok = std::fread(off.data(), sizeof(int64_t), off.size(), f) == off.size() &&
        std::fread(m.data(), sizeof(double), m.size(), f) == m.size() &&
        std::fread(rcv.data(), sizeof(double), rcv.size(), f) == rcv.size() &&
+       std::fread(rad.data(), sizeof(double), rad.size(), f) == rad.size() &&
        std::fread(mx.data(), sizeof(double), mx.size(), f) == mx.size() &&
        std::fread(mv.data(), sizeof(double), mv.size(), f) == mv.size();
   for (int k = 0; k < 6 && ok; ++k) ok = std::fread(t[k].data(), sizeof(double), (size_t)N, f) == (size_t)N;
@@ -585,12 +610,37 @@ int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
     if (N > 0 && (rc = c->hip(hipMemcpy(c->d_tp[k], t[k].data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice),
                               "H2D")))
       return rc;
+  if ((rc = mint_set_radii(ctx, rad.data()))) return rc;
   if (step_index) *step_index = hdr_l[1];
   return MINT_OK;
 }
 
+int mint_set_radii(void *ctx, const double *radius) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_set_radii: nothing uploaded");
+  if (!radius) return c->fail(MINT_EINVAL, "mint_set_radii: null radii");
+  const size_t n = (size_t)c->S * c->NM;
+  for (size_t i = 0; i < n; ++i)
+    if (!(radius[i] >= 0.0) || !std::isfinite(radius[i]))
+      return c->fail(MINT_EINVAL, "mint_set_radii: radii must be finite and >= 0");
+  hipSetDevice(c->cfg.device);
+  int rc = c->hip(hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  c->rad.assign(radius, radius + n);
+  if (c->many) return mb_set_radii(c);
+  const int PS = prm_size(c->NM);
+  for (int s = 0; s < c->S; ++s)
+    if ((rc = c->hip(hipMemcpy(c->d_prm + (size_t)s * PS + P_RAD(c->NM), radius + (size_t)s * c->NM,
+                               (size_t)c->NM * sizeof(double), hipMemcpyHostToDevice),
+                     "H2D radii")))
+      return rc;
+  return MINT_OK;
+}
+
 // E = sum_j m_j |v_j|^2/2 + |sum_j m_j v_j|^2/(2M) - sum_j M m_j/|x_j|
-//     - sum_{j<k} m_j m_k/|x_j - x_k|   (democratic-heliocentric Hamiltonian)
+//     + sum_{j<k} m_j m_k U(|x_j - x_k|)   (democratic-heliocentric Hamiltonian),
+// U(r) = -atan(eps/r)/eps, the potential of the softened pair force.
 int mint_energy(void *ctx, double *E) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
@@ -605,6 +655,7 @@ int mint_energy(void *ctx, double *E) {
   if ((rc = c->hip(hipMemcpy(mx.data(), c->d_mx, mb, hipMemcpyDeviceToHost), "D2H massive x"))) return rc;
   if ((rc = c->hip(hipMemcpy(mv.data(), c->d_mv, mb, hipMemcpyDeviceToHost), "D2H massive v"))) return rc;
   const double M = c->cfg.central_mass;
+  const double eps = std::sqrt(c->cfg.softening2);
   for (int s = 0; s < S; ++s) {
     const double *x = mx.data() + (size_t)s * NM * 3;
     const double *v = mv.data() + (size_t)s * NM * 3;
@@ -618,7 +669,9 @@ int mint_energy(void *ctx, double *E) {
       pot -= M * m[j] / std::sqrt(dot3h(x + 3 * j, x + 3 * j));
       for (int k = j + 1; k < NM; ++k) {
         const double d[3] = {x[3 * k] - x[3 * j], x[3 * k + 1] - x[3 * j + 1], x[3 * k + 2] - x[3 * j + 2]};
-        pot -= m[j] * m[k] / std::sqrt(dot3h(d, d));
+        // potential of the softened pair force m_j m_k d/((r^2+eps^2) r)
+        const double r = std::sqrt(dot3h(d, d));
+        pot += m[j] * m[k] * (eps > 0.0 ? -std::atan(eps / r) / eps : -1.0 / r);
       }
     }
     const double p2 = std::fma(px, px, std::fma(py, py, pz * pz));
@@ -656,6 +709,9 @@ int mint_get_stats(void *ctx, mint_stats *out) {
   out->prebucketed = (int64_t)v[S_PREBUCKET];
   out->idle_lane_steps = (int64_t)v[S_IDLE];
   out->enc_tp_steps = (int64_t)v[S_ENCSTEPS];
+  out->massive_groups = (int64_t)v[S_MGROUPS];
+  out->massive_group_bodies = (int64_t)v[S_MGBODIES];
+  out->overlaps = (int64_t)v[S_OVERLAP];
   return MINT_OK;
 }
 

@@ -66,6 +66,19 @@ struct Ctx {
   std::vector<GraphEntry> graphs;
   mint_timing timing{};
 
+  // many-body engine state (n_massive > MAX_NM, no TPs)
+  bool many = false;
+  std::vector<double> rad;          // physical radii [S*NM] (zeros: none)
+  double *d_bp = nullptr;           // [S][5][NM]: m, 0.1 r_c, 1/(0.9 r_c), r_c^2, R
+  double *d_mxb = nullptr, *d_mvb = nullptr;  // post-drift x, v  [S][NM][3]
+  double *d_my = nullptr;           // post-jump positions (drift start) [S][NM][3]
+  int *d_mflag = nullptr;           // [S][NM] body is in an encounter pair this step
+  int *d_medge = nullptr;           // [S][MB_EMAX][2] encounter pairs
+  unsigned int *d_mctl = nullptr;   // [S][4] pairs, blocks done, groups, grouped bodies
+  int *d_gstart = nullptr, *d_gmem = nullptr;  // [S][NM+1] group starts, [S][NM] members
+  hipGraphExec_t mb_graph = nullptr;
+  long long mb_graph_steps = 0;
+
   int fail(int code, const std::string &msg);
   int hip(hipError_t e, const char *what);
   void free_device();
@@ -76,6 +89,47 @@ struct Ctx {
   StepConst step_const() const;
   TPArrays tp_arrays() const;
 };
+
+inline bool timing_on(const Ctx *c) { return (c->cfg.collect_stats & 2) != 0; }
+
+// HIP-event timer around single launches on the context stream (profiling
+// runs only: stop() synchronises).
+struct KTimer {
+  Ctx *c;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool on;
+  explicit KTimer(Ctx *cc) : c(cc), on(timing_on(cc)) {
+    if (on) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+    }
+  }
+  ~KTimer() {
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+  }
+  void start() {
+    if (on) hipEventRecord(a, c->stream);
+  }
+  void stop(double &ms, int64_t &n) {
+    if (!on) return;
+    hipEventRecord(b, c->stream);
+    hipEventSynchronize(b);
+    float t = 0.f;
+    hipEventElapsedTime(&t, a, b);
+    ms += t;
+    n += 1;
+  }
+};
+
+// many-body engine (mint_mb.hip): systems of more than MAX_NM massive bodies
+constexpr int MB_MAXN = 2048;   // massive bodies per system
+constexpr int MB_EMAX = 2048;   // encounter pairs per system and step
+constexpr int MB_GMAX = 64;     // bodies per encounter group (one wavefront)
+int mb_alloc(Ctx *c);       // engine buffers for the uploaded shape
+int mb_reset(Ctx *c);       // body parameters from m, rc, rad; clear flags and counters
+int mb_set_radii(Ctx *c);   // body parameters again after a change of rad
+int run_many(Ctx *c, int64_t nsteps);
 
 extern std::string g_create_error;
 const char *form_name(int f);

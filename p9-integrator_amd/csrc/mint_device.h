@@ -19,6 +19,15 @@ constexpr int BS_MAXDEPTH = 24;    // interval halvings before giving up
 constexpr int NEWTON_MAXIT = 40;   // Kepler Newton iteration cap
 
 constexpr uint32_t ST_NAN = 1u, ST_NEWTON = 2u, ST_BS_DEPTH = 4u, ST_MASSIVE_ENC = 8u;
+constexpr uint32_t ST_OVERLAP = 16u, ST_LIMIT = 32u;
+
+// Sums over massive bodies run in chunks of MB_CHUNK bodies in index order:
+// each chunk's sum starts from zero and the chunk sums are added in chunk
+// order.  Kick weights are divided in groups of MB_DIVG bodies with one
+// division per group (batched_div), the body itself an inert slot.  Systems of
+// up to 8 massive bodies are one chunk and one division group.
+constexpr int MB_CHUNK = 16;
+constexpr int MB_DIVG = 8;
 
 // Per-system parameter block (doubles), NM massive bodies:
 //   [0,NM)           m_j
@@ -28,7 +37,8 @@ constexpr uint32_t ST_NAN = 1u, ST_NEWTON = 2u, ST_BS_DEPTH = 4u, ST_MASSIVE_ENC
 //   [4NM, 4NM+NM^2)          pair 0.1 r_c   massive-massive, r_c = max(r_c,j, r_c,k)
 //   [4NM+NM^2, 4NM+2NM^2)    pair 1/(0.9 r_c)
 //   [4NM+2NM^2, 4NM+3NM^2)   pair r_c^2
-__host__ __device__ constexpr int prm_size(int nm) { return ((4 * nm + 3 * nm * nm) + 1) & ~1; }
+//   [4NM+3NM^2, 5NM+3NM^2)   physical radius R_j (0 = none)
+__host__ __device__ constexpr int prm_size(int nm) { return ((5 * nm + 3 * nm * nm) + 1) & ~1; }
 __host__ __device__ constexpr int P_M(int) { return 0; }
 __host__ __device__ constexpr int P_RCLO(int nm) { return nm; }
 __host__ __device__ constexpr int P_RCWINV(int nm) { return 2 * nm; }
@@ -36,6 +46,7 @@ __host__ __device__ constexpr int P_RC2(int nm) { return 3 * nm; }
 __host__ __device__ constexpr int P_PRCLO(int nm) { return 4 * nm; }
 __host__ __device__ constexpr int P_PRCWINV(int nm) { return 4 * nm + nm * nm; }
 __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; }
+__host__ __device__ constexpr int P_RAD(int nm) { return 4 * nm + 3 * nm * nm; }
 
 // Perturber cache: for every system and step, the massive positions
 // Kepler-drifted from (Y, W) by tau = m (h/16), m = 1..16, the substep times of
@@ -517,35 +528,40 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
 
 // Kick acceleration of every massive body from the others, pair changeover
 // radius max(r_c,j, r_c,k) (body_interaction_kick over massive partners,
-// src/sim.cuh:327-382, with self-exclusion by body index).
+// src/sim.cuh:327-382, with self-exclusion by body index: the body's own slot
+// has numerator 0, denominator 1 and separation 0, which changes no bit of the
+// other weights or of the sum).
 template <int NM>
 __device__ __forceinline__ void accel_massive(const double *X, const double *__restrict__ prm,
                                               double eps2, double *A) {
+  static_assert(NM <= MB_DIVG, "one division group");
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
+    double dx[NM], dy[NM], dz[NM], num[NM], den[NM], w[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      if (k == j) {
+        dx[k] = 0.0;
+        dy[k] = 0.0;
+        dz[k] = 0.0;
+        num[k] = 0.0;
+        den[k] = 1.0;
+        continue;
+      }
+      dx[k] = X[3 * k + 0] - X[3 * j + 0];
+      dy[k] = X[3 * k + 1] - X[3 * j + 1];
+      dz[k] = X[3 * k + 2] - X[3 * j + 2];
+      const double r2 = dot3(dx[k], dy[k], dz[k], dx[k], dy[k], dz[k]);
+      kick_nd(r2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
+              prm[P_PRCWINV(NM) + j * NM + k], num[k], den[k]);
+    }
+    batched_div<NM>(num, den, w);
     double ax = 0.0, ay = 0.0, az = 0.0;
-    if (NM > 1) {
-      constexpr int NP = NM > 1 ? NM - 1 : 1;
-      double dx[NP], dy[NP], dz[NP], num[NP], den[NP], w[NP];
-      int q = 0;
 #pragma unroll
-      for (int k = 0; k < NM; ++k) {
-        if (k == j) continue;
-        dx[q] = X[3 * k + 0] - X[3 * j + 0];
-        dy[q] = X[3 * k + 1] - X[3 * j + 1];
-        dz[q] = X[3 * k + 2] - X[3 * j + 2];
-        const double r2 = dot3(dx[q], dy[q], dz[q], dx[q], dy[q], dz[q]);
-        kick_nd(r2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
-                prm[P_PRCWINV(NM) + j * NM + k], num[q], den[q]);
-        ++q;
-      }
-      batched_div<NP>(num, den, w);
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        ax = fma(w[p], dx[p], ax);
-        ay = fma(w[p], dy[p], ay);
-        az = fma(w[p], dz[p], az);
-      }
+    for (int k = 0; k < NM; ++k) {
+      ax = fma(w[k], dx[k], ax);
+      ay = fma(w[k], dy[k], ay);
+      az = fma(w[k], dz[k], az);
     }
     A[3 * j + 0] = ax;
     A[3 * j + 1] = ay;
@@ -570,9 +586,15 @@ __device__ __forceinline__ void momentum_q(const double *V, const double *__rest
   Q[2] = pz / M;
 }
 
+// Encounter graph of the massive bodies at X: adj[j] has bit k set when the
+// pair (j, k) is inside its changeover radius.  Also counts the pairs closer
+// than the sum of their physical radii.  Returns whether any edge exists.
 template <int NM>
-__device__ __forceinline__ bool encounter_massive(const double *X, const double *__restrict__ prm) {
-  bool f = false;
+__device__ __forceinline__ bool encounter_graph(const double *X, const double *__restrict__ prm,
+                                                uint32_t *adj, unsigned int &overlaps) {
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) adj[j] = 0u;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
 #pragma unroll
@@ -581,28 +603,38 @@ __device__ __forceinline__ bool encounter_massive(const double *X, const double 
       const double dy = X[3 * k + 1] - X[3 * j + 1];
       const double dz = X[3 * k + 2] - X[3 * j + 2];
       const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-      f = f || (r2 < prm[P_PRC2(NM) + j * NM + k]);
+      const double sr = prm[P_RAD(NM) + j] + prm[P_RAD(NM) + k];
+      if (r2 < sr * sr) ++overlaps;
+      if (r2 < prm[P_PRC2(NM) + j * NM + k]) {
+        adj[j] |= 1u << k;
+        adj[k] |= 1u << j;
+        any = true;
+      }
     }
   }
-  return f;
+  return any;
 }
 
-// Close-encounter RHS of the whole massive set (state [NM][6]).
+// Close-encounter RHS of one encounter group (g members, body indices idx[],
+// state [g][6]): the Sun plus the (1-K)-weighted pulls of the other members.
 template <int NM>
-__device__ __forceinline__ void rhs_massive(const double *s, const double *__restrict__ prm,
-                                            double mu, double eps2, double *f) {
-  for (int j = 0; j < NM; ++j) {
-    const double x = s[6 * j + 0], y = s[6 * j + 1], z = s[6 * j + 2];
-    f[6 * j + 0] = s[6 * j + 3];
-    f[6 * j + 1] = s[6 * j + 4];
-    f[6 * j + 2] = s[6 * j + 5];
+__device__ __forceinline__ void rhs_group(const double *s, int g, const int *idx,
+                                          const double *__restrict__ prm, double mu, double eps2,
+                                          double *f) {
+  for (int a = 0; a < g; ++a) {
+    const int j = idx[a];
+    const double x = s[6 * a + 0], y = s[6 * a + 1], z = s[6 * a + 2];
+    f[6 * a + 0] = s[6 * a + 3];
+    f[6 * a + 1] = s[6 * a + 4];
+    f[6 * a + 2] = s[6 * a + 5];
     const double r2 = dot3(x, y, z, x, y, z);
     const double r = sqrt(r2);
     const double w0 = mu / (r2 * r);
     double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
-    for (int k = 0; k < NM; ++k) {
-      if (k == j) continue;
-      const double dx = s[6 * k + 0] - x, dy = s[6 * k + 1] - y, dz = s[6 * k + 2] - z;
+    for (int b = 0; b < g; ++b) {
+      if (b == a) continue;
+      const int k = idx[b];
+      const double dx = s[6 * b + 0] - x, dy = s[6 * b + 1] - y, dz = s[6 * b + 2] - z;
       const double d2 = dot3(dx, dy, dz, dx, dy, dz);
       const double w = close_weight(d2, eps2, prm[P_M(NM) + k], prm[P_PRCLO(NM) + j * NM + k],
                                     prm[P_PRCWINV(NM) + j * NM + k]);
@@ -610,72 +642,76 @@ __device__ __forceinline__ void rhs_massive(const double *s, const double *__res
       ay = fma(w, dy, ay);
       az = fma(w, dz, az);
     }
-    f[6 * j + 3] = ax;
-    f[6 * j + 4] = ay;
-    f[6 * j + 5] = az;
+    f[6 * a + 3] = ax;
+    f[6 * a + 4] = ay;
+    f[6 * a + 5] = az;
   }
 }
 
 template <int NM>
-__device__ __noinline__ void gragg_massive(const double *y0, const double *f0, double H, int N,
-                                           const double *__restrict__ prm, double mu,
-                                           double eps2, double *out) {
+__device__ __noinline__ void gragg_group(const double *y0, const double *f0, double H, int N, int g,
+                                         const int *idx, const double *__restrict__ prm,
+                                         double mu, double eps2, double *out) {
   constexpr int D = 6 * NM;
+  const int n = 6 * g;
   const double hs = H / (double)N;
   const double h2 = 2.0 * hs;
   double z0[D], z1[D], f[D];
-  for (int i = 0; i < D; ++i) {
+  for (int i = 0; i < n; ++i) {
     z0[i] = y0[i];
     z1[i] = fma(hs, f0[i], y0[i]);
   }
   for (int m = 1; m < N; ++m) {
-    rhs_massive<NM>(z1, prm, mu, eps2, f);
-    for (int i = 0; i < D; ++i) {
+    rhs_group<NM>(z1, g, idx, prm, mu, eps2, f);
+    for (int i = 0; i < n; ++i) {
       const double z2 = fma(h2, f[i], z0[i]);
       z0[i] = z1[i];
       z1[i] = z2;
     }
   }
-  rhs_massive<NM>(z1, prm, mu, eps2, f);
-  for (int i = 0; i < D; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
+  rhs_group<NM>(z1, g, idx, prm, mu, eps2, f);
+  for (int i = 0; i < n; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
 }
 
-// BS drift of the whole massive set when any massive pair is inside its
-// changeover radius.  Error scale per body: |x_j|, |v_j|.
+// BS drift of one massive encounter group over h.  Error scale per member:
+// |x_j|, |v_j| at the start of each interval.
 template <int NM>
-__device__ __noinline__ uint32_t bs_drift_massive(double *s, double h,
-                                                  const double *__restrict__ prm, double mu,
-                                                  double eps2, double tol) {
+__device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx, double h,
+                                                const double *__restrict__ prm, double mu,
+                                                double eps2, double tol, unsigned int &nrhs) {
   constexpr int D = 6 * NM;
+  const int n = 6 * g;
   uint32_t st = 0;
   double t = 0.0, H = h;
   int depth = 0;
   while (t < h) {
     if (H > h - t) H = h - t;
     double f0[D], sc[2 * NM];
-    rhs_massive<NM>(s, prm, mu, eps2, f0);
-    for (int j = 0; j < NM; ++j) {
-      sc[2 * j] = sqrt(dot3(s[6 * j], s[6 * j + 1], s[6 * j + 2], s[6 * j], s[6 * j + 1], s[6 * j + 2]));
-      sc[2 * j + 1] =
-          sqrt(dot3(s[6 * j + 3], s[6 * j + 4], s[6 * j + 5], s[6 * j + 3], s[6 * j + 4], s[6 * j + 5]));
+    rhs_group<NM>(s, g, idx, prm, mu, eps2, f0);
+    ++nrhs;
+    for (int a = 0; a < g; ++a) {
+      sc[2 * a] = sqrt(dot3(s[6 * a], s[6 * a + 1], s[6 * a + 2], s[6 * a], s[6 * a + 1], s[6 * a + 2]));
+      sc[2 * a + 1] =
+          sqrt(dot3(s[6 * a + 3], s[6 * a + 4], s[6 * a + 5], s[6 * a + 3], s[6 * a + 4], s[6 * a + 5]));
     }
     double R[KMAX][D];
     double cur[D];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
-      gragg_massive<NM>(s, f0, H, 2 << k, prm, mu, eps2, cur);
+      gragg_group<NM>(s, f0, H, 2 << k, g, idx, prm, mu, eps2, cur);
+      nrhs += 2u << k;
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
-        for (int i = 0; i < D; ++i) {
+        for (int i = 0; i < n; ++i) {
           const double prev = R[j - 1][i];
           R[j - 1][i] = cur[i];
           cur[i] = cur[i] + (cur[i] - prev) / fac;
         }
       }
-      for (int i = 0; i < D; ++i) R[k][i] = cur[i];
+      for (int i = 0; i < n; ++i) R[k][i] = cur[i];
       if (k >= 1) {
         double err = 0.0;
-        for (int i = 0; i < D; ++i) {
+        for (int i = 0; i < n; ++i) {
           const int b = i / 6, c = i % 6;
           const double e = fabs(cur[i] - R[k - 1][i]) / sc[2 * b + (c < 3 ? 0 : 1)];
           err = e > err ? e : err;
@@ -688,7 +724,7 @@ __device__ __noinline__ uint32_t bs_drift_massive(double *s, double h,
     }
     if (conv || depth >= BS_MAXDEPTH) {
       if (!conv) st |= ST_BS_DEPTH;
-      for (int i = 0; i < D; ++i) s[i] = cur[i];
+      for (int i = 0; i < n; ++i) s[i] = cur[i];
       t += H;
     } else {
       H *= 0.5;

@@ -47,7 +47,10 @@ enum {
   S_PREBUCKET = 6,
   S_IDLE = 7,
   S_ENCSTEPS = 8,
-  S_COUNT = 9
+  S_MGROUPS = 9,
+  S_MGBODIES = 10,
+  S_OVERLAP = 11,
+  S_COUNT = 12
 };
 
 __device__ __forceinline__ void q_put(QEntry *e, int tp, int sys, int k0, int phase, double x,
@@ -129,6 +132,7 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
                                                        double *__restrict__ xs0,
                                                        double *__restrict__ rec,
                                                        unsigned int *__restrict__ status,
+                                                       unsigned long long *__restrict__ stats,
                                                        StepConst sc, double M) {
   constexpr int REC = rec_size(NM);
   const int s = s_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -160,28 +164,65 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
       r[R_Y(NM) + i] = X[i];
       r[R_W(NM) + i] = V[i];
     }
-    if (encounter_massive<NM>(X, prm)) {
+    // encounter groups: connected components of the changeover graph, each
+    // drifted by BS as one system; every other body drifts on its Kepler orbit
+    uint32_t adj[NM];
+    unsigned int ov = 0;
+    uint32_t ingrp = 0u;
+    if (encounter_graph<NM>(X, prm, adj, ov)) {
       st |= ST_MASSIVE_ENC;
-      double sv[6 * NM];
-      for (int j = 0; j < NM; ++j)
-        for (int c = 0; c < 3; ++c) {
-          sv[6 * j + c] = X[3 * j + c];
-          sv[6 * j + 3 + c] = V[3 * j + c];
-        }
-      st |= bs_drift_massive<NM>(sv, sc.h, prm, sc.mu, sc.eps2, sc.tol);
-      for (int j = 0; j < NM; ++j)
-        for (int c = 0; c < 3; ++c) {
-          X[3 * j + c] = sv[6 * j + c];
-          V[3 * j + c] = sv[6 * j + 3 + c];
-        }
-    } else {
+      unsigned int ng = 0, nb = 0, nr = 0;
 #pragma unroll
-      for (int j = 0; j < NM; ++j) {
-        bool ok;
-        kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
-                            V[3 * j + 2], sc.mu, sc.h, ok);
-        if (!ok) st |= ST_NEWTON;
+      for (int r = 0; r < NM; ++r) {
+        if (adj[r] == 0u || ((ingrp >> r) & 1u)) continue;
+        uint32_t comp = 1u << r, front = comp;
+        while (front) {
+          uint32_t nxt = 0u;
+#pragma unroll
+          for (int j = 0; j < NM; ++j)
+            if ((front >> j) & 1u) nxt |= adj[j];
+          front = nxt & ~comp;
+          comp |= nxt;
+        }
+        ingrp |= comp;
+        int idx[NM], g = 0;
+        double sv[6 * NM];
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+          if (!((comp >> j) & 1u)) continue;
+          for (int c = 0; c < 3; ++c) {
+            sv[6 * g + c] = X[3 * j + c];
+            sv[6 * g + 3 + c] = V[3 * j + c];
+          }
+          idx[g++] = j;
+        }
+        st |= bs_drift_group<NM>(sv, g, idx, sc.h, prm, sc.mu, sc.eps2, sc.tol, nr);
+        for (int a = 0; a < g; ++a) {
+          const int j = idx[a];
+          for (int c = 0; c < 3; ++c) {
+            X[3 * j + c] = sv[6 * a + c];
+            V[3 * j + c] = sv[6 * a + 3 + c];
+          }
+        }
+        ++ng;
+        nb += (unsigned int)g;
       }
+      if (stats) {
+        atomicAdd(stats + S_MGROUPS, (unsigned long long)ng);
+        atomicAdd(stats + S_MGBODIES, (unsigned long long)nb);
+      }
+    }
+    if (ov) {
+      st |= ST_OVERLAP;
+      if (stats) atomicAdd(stats + S_OVERLAP, (unsigned long long)ov);
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      if ((ingrp >> j) & 1u) continue;
+      bool ok;
+      kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
+                          V[3 * j + 2], sc.mu, sc.h, ok);
+      if (!ok) st |= ST_NEWTON;
     }
     momentum_q<NM>(V, prm, M, Q);
 #pragma unroll

@@ -10,7 +10,8 @@ namespace mint {
 struct Launchers {
   hipError_t (*ephemeris)(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
                           double *mx, double *mv, double *xs0, double *rec,
-                          unsigned int *status, StepConst sc, double M);
+                          unsigned int *status, unsigned long long *stats, StepConst sc,
+                          double M);
   hipError_t (*tp_step)(hipStream_t st, int guess, int block, long long i0, long long i1,
                         const int *tp_sys, TPArrays tp, const double *prm, const double *xs0,
                         const double *rec, StepConst sc, const float *nnw,

@@ -12,11 +12,11 @@ constexpr int NMV = MINT_NM;
 
 hipError_t ephemeris_l(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
                        double *mx, double *mv, double *xs0, double *rec, unsigned int *status,
-                       StepConst sc, double M) {
+                       unsigned long long *stats, StepConst sc, double M) {
   const int n = s_end - s_begin;
   if (n <= 0) return hipSuccess;
   ephemeris_kernel<NMV><<<(n + 63) / 64, 64, 0, st>>>(s_begin, s_end, C, prm, mx, mv, xs0, rec,
-                                                      status, sc, M);
+                                                      status, stats, sc, M);
   return hipGetLastError();
 }
 

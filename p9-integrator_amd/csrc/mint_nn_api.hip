@@ -348,7 +348,7 @@ int mint_nn_set_weights(void *ctx, int32_t which, const float *w) {
   for (auto &g : c->graphs)
     if (g.exec) hipGraphExecDestroy(g.exec);
   c->graphs.clear();
-  if (c->uploaded) {
+  if (c->uploaded && dev_weights(c, which)) {  // (the many-body engine keeps no network buffers)
     hipSetDevice(c->cfg.device);
     return c->hip(hipMemcpy(dev_weights(c, which), hw.data(), hw.size() * sizeof(float), hipMemcpyHostToDevice),
                   "H2D weights");
@@ -370,6 +370,7 @@ int mint_trace_kepler(void *ctx, int64_t nsteps, int64_t cap, float *feat, float
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
   if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_trace_kepler: nothing uploaded");
+  if (c->many) return c->fail(MINT_EINVAL, "mint_trace_kepler: needs test particles (at most 8 massive bodies)");
   if (cap <= 0 || cap > 0x7fffffff || !feat || !target || !n_out || nsteps < 1)
     return c->fail(MINT_EINVAL, "mint_trace_kepler: bad arguments");
   hipSetDevice(c->cfg.device);
@@ -419,6 +420,7 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
   if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_trace_encounter: nothing uploaded");
+  if (c->many) return c->fail(MINT_EINVAL, "mint_trace_encounter: needs test particles (at most 8 massive bodies)");
   if (cap <= 0 || cap > 0x7fffffff || !feat || !label || !n_out || nsteps < 1)
     return c->fail(MINT_EINVAL, "mint_trace_encounter: bad arguments");
   if (c->NM != 5 && c->NM != 2)
@@ -443,7 +445,7 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
     hipMemcpyAsync(cmx.p, c->d_mx, mb, hipMemcpyDeviceToDevice, c->stream);
     hipMemcpyAsync(cmv.p, c->d_mv, mb, hipMemcpyDeviceToDevice, c->stream);
     rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, C, c->d_prm, (double *)cmx.p, (double *)cmv.p, c->d_xs0,
-                                c->d_rec, c->d_status, sc, c->cfg.central_mass),
+                                c->d_rec, c->d_status, nullptr, sc, c->cfg.central_mass),
                 "trace ephemeris");
     if (rc) return rc;
     const long long cnt = (c->Ntp + stride - 1) / stride;
@@ -578,7 +580,7 @@ int mint_train(void *ctx, int32_t which, int64_t n, const float *feat, const flo
   for (auto &g : c->graphs)
     if (g.exec) hipGraphExecDestroy(g.exec);
   c->graphs.clear();
-  if (c->uploaded)
+  if (c->uploaded && dev_weights(c, which))
     hipMemcpy(dev_weights(c, which), hw.data(), np * 4, hipMemcpyHostToDevice);
   return c->hip(hipGetLastError(), "train");
 }

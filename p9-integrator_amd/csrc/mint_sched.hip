@@ -43,43 +43,15 @@ int enc_grid(const Ctx *c) {
   return (int)std::max(1LL, std::min(g, 2048LL));
 }
 
-bool timing_on(const Ctx *c) { return (c->cfg.collect_stats & 2) != 0; }
 
-struct KTimer {
-  Ctx *c;
-  hipEvent_t a = nullptr, b = nullptr;
-  bool on;
-  explicit KTimer(Ctx *cc) : c(cc), on(timing_on(cc)) {
-    if (on) {
-      hipEventCreate(&a);
-      hipEventCreate(&b);
-    }
-  }
-  ~KTimer() {
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
-  }
-  void start() {
-    if (on) hipEventRecord(a, c->stream);
-  }
-  // stop and add to an accumulator (synchronises; profiling runs only)
-  void stop(double &ms, int64_t &n) {
-    if (!on) return;
-    hipEventRecord(b, c->stream);
-    hipEventSynchronize(b);
-    float t = 0.f;
-    hipEventElapsedTime(&t, a, b);
-    ms += t;
-    n += 1;
-  }
-};
 
 // Ephemeris of one launch window into buffer (xs0, rec) on stream st.
 int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, KTimer *kt) {
   const StepConst sc = c->step_const();
   if (kt) kt->start();
-  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, c->d_status, sc,
-                                  c->cfg.central_mass),
+  unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
+  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, c->d_status, stats,
+                                  sc, c->cfg.central_mass),
                   "ephemeris launch");
   if (rc) return rc;
   if (kt) kt->stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
@@ -277,7 +249,7 @@ int run_per_step(Ctx *c, int64_t nsteps, bool flat) {
       const int s1 = std::min(c->S, s0 + spl);
       kt.start();
       rc = c->hip(c->L->ephemeris(c->stream, s0, s1, 1, c->d_prm, c->d_mx, c->d_mv, c->d_xs0, c->d_rec,
-                                  c->d_status, sc, c->cfg.central_mass),
+                                  c->d_status, stats, sc, c->cfg.central_mass),
                   "ephemeris launch");
       if (rc) return rc;
       kt.stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
@@ -296,6 +268,7 @@ int run_per_step(Ctx *c, int64_t nsteps, bool flat) {
 }  // namespace
 
 int step_impl(Ctx *c, int64_t nsteps) {
+  if (c->many) return run_many(c, nsteps);
   switch (c->cfg.form) {
     case MINT_FORM_PLAIN: return run_per_step(c, nsteps, false);
     case MINT_FORM_FLAT: return run_per_step(c, nsteps, true);
@@ -343,6 +316,10 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
   if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_autotune: nothing uploaded");
+  if (c->many) {  // the many-body engine has a fixed launch shape
+    if (best) best[0] = best[1] = best[2] = 0;
+    return MINT_OK;
+  }
   hipSetDevice(c->cfg.device);
   hipDeviceProp_t prop;
   hipGetDeviceProperties(&prop, c->cfg.device);

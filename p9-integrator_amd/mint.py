@@ -23,7 +23,9 @@ LIB_PATH = os.path.join(HERE, "libmint.so")
 
 FORM_PLAIN, FORM_FLAT, FORM_BATCHED = 0, 1, 2
 GUESS_CRUDE, GUESS_SERIES, GUESS_LEARNED = 0, 1, 2
-ST_NAN, ST_NEWTON, ST_BS_DEPTH, ST_MASSIVE_ENC = 1, 2, 4, 8
+ST_NAN, ST_NEWTON, ST_BS_DEPTH, ST_MASSIVE_ENC, ST_OVERLAP, ST_LIMIT = 1, 2, 4, 8, 16, 32
+MAX_NM_TP = 8         # massive bodies per system when the system has test particles
+MAX_NM = 2048         # massive bodies per system without test particles
 NET_KEPLER, NET_SCORER = 0, 1
 
 
@@ -48,6 +50,7 @@ class _Config(C.Structure):
         ("use_graph", C.c_int32),
         ("device", C.c_int32),
         ("collect_stats", C.c_int32),
+        ("many_body", C.c_int32),
     ]
 
 
@@ -75,7 +78,8 @@ class _State(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("tp_steps", "kepler_solves", "newton_iters", "bs_steps",
                                          "bs_rhs_evals", "deferred", "prebucketed", "idle_lane_steps",
-                                         "enc_tp_steps")]
+                                         "enc_tp_steps", "massive_groups", "massive_group_bodies",
+                                         "overlaps")]
 
 
 class _Timing(C.Structure):
@@ -84,6 +88,9 @@ class _Timing(C.Structure):
         ("ms_tp_main", C.c_double), ("n_tp_main", C.c_int64),
         ("ms_encounter", C.c_double), ("n_encounter", C.c_int64),
         ("ms_total", C.c_double),
+        ("ms_mb_open", C.c_double), ("n_mb_open", C.c_int64),
+        ("ms_mb_drift", C.c_double), ("n_mb_drift", C.c_int64),
+        ("ms_mb_close", C.c_double), ("n_mb_close", C.c_int64),
     ]
 
 
@@ -123,7 +130,11 @@ _SIGS = {
     "mint_set_score_threshold": (C.c_int, [C.c_void_p, C.c_float]),
     "mint_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "mint_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
+    "mint_set_radii": (C.c_int, [C.c_void_p, _DP]),
 }
+
+
+ABI_VERSION = 2
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -134,6 +145,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
+    if lib.mint_abi_version() != ABI_VERSION:
+        raise MintError(f"{path} has ABI {lib.mint_abi_version()}, this mirror binds ABI {ABI_VERSION}; rebuild")
     return lib
 
 
@@ -218,7 +231,8 @@ class Mint:
 
     def __init__(self, dt=0.8219, central_mass=1.0, n_hill=3.0, softening2=1e-8, bs_tol=1e-12,
                  form=FORM_BATCHED, kepler_guess=GUESS_SERIES, systems_per_launch=4, block_size=0,
-                 tps_per_lane=0, chunk_steps=0, use_scorer=0, use_graph=1, device=0, collect_stats=0):
+                 tps_per_lane=0, chunk_steps=0, use_scorer=0, use_graph=1, device=0, collect_stats=0,
+                 many_body=0):
         self.L = lib()
         cfg = _Config()
         self.L.mint_config_default(C.byref(cfg))
@@ -226,7 +240,7 @@ class Mint:
                          form=form, kepler_guess=kepler_guess, systems_per_launch=systems_per_launch,
                          block_size=block_size, tps_per_lane=tps_per_lane, chunk_steps=chunk_steps,
                          use_scorer=use_scorer, use_graph=use_graph, device=device,
-                         collect_stats=collect_stats).items():
+                         collect_stats=collect_stats, many_body=many_body).items():
             setattr(cfg, k, v)
         self.cfg = cfg
         h = C.c_void_p()
@@ -287,14 +301,21 @@ class Mint:
     def load_checkpoint(self, path) -> int:
         k = C.c_int64()
         self._chk(self.L.mint_load_checkpoint(self.h, path.encode(), C.byref(k)), "mint_load_checkpoint")
-        S = self.L  # shape comes from the file; read it back through a download
-        del S
+        # the shape comes from the file header: magic, int32 S, NM, version, int64 Ntp, step
         hdr = np.fromfile(path, dtype=np.int32, count=5)[2:5]
         n = int(np.fromfile(path, dtype=np.int64, count=1, offset=20)[0])
         off = np.fromfile(path, dtype=np.int64, count=int(hdr[0]) + 1, offset=20 + 16 + 40)
         self._shape = (int(hdr[0]), int(hdr[1]), off)
         assert int(off[-1]) == n
         return k.value
+
+    def set_radii(self, radius):
+        """Physical radii [S, NM] (recorded and checked for overlaps; nothing merges)."""
+        r = np.ascontiguousarray(radius, np.float64).ravel()
+        S, NM = self._shape[0], self._shape[1]
+        if r.size != S * NM:
+            raise MintError(f"set_radii: need {S * NM} radii, got {r.size}")
+        self._chk(self.L.mint_set_radii(self.h, _d(r)), "mint_set_radii")
 
     def energy(self) -> np.ndarray:
         E = np.empty(self._shape[0])

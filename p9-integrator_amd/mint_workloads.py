@@ -8,6 +8,7 @@ the library's host conversion (mint_elements_to_cartesian).
   earth_mars()      Sun + Earth-like + Mars-like, no TPs            (dt 1e-3)
   p9_single(n_tp)   Sun + 4 giants + Planet Nine + n_tp TPs         (dt 0.5)
   p9_sweep(S)       S independent P9 systems, lognormal TP counts   (dt 0.5)
+  embryos(n)        Sun + n planetary embryos, all massive          (dt 0.002)
 """
 from __future__ import annotations
 
@@ -22,6 +23,9 @@ GIANT_A = np.array([5.2, 9.55, 19.2, 30.1])
 SWEEP_SIGMA = 0.75                  # lognormal sigma of TPs per system (fixed here)
 SWEEP_MEDIAN = 1024
 SWEEP_CLIP = (256, 8192)
+AU_CM = 1.495978707e13
+MSUN_G = 1.98847e33
+EMBRYO_DENSITY = 3.0                # g/cm^3
 
 
 def _rng(*key):
@@ -140,3 +144,27 @@ def inner_belt(n_tp=20000, a_range=(1.5, 3.0), seed_key=0) -> mint.State:
     tx, tv = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], e, a)
     return mint.State(np.array([0, n_tp], np.int64), base.m, base.mx, base.mv, tx, tv,
                       meta=dict(workload="inner_belt", dt=0.5))
+
+
+def embryo_radii(m, density=EMBRYO_DENSITY):
+    """Physical radius [AU] of a body of mass m [Msun] at a density [g/cm^3]."""
+    rho = density * AU_CM ** 3 / MSUN_G          # Msun / AU^3
+    return np.cbrt(3.0 * np.asarray(m, float) / (4.0 * np.pi * rho))
+
+
+def embryos(n=1000, seed_key=0) -> mint.State:
+    """Sun + n planetary embryos (encounter-heavy, all massive): m = 3e-7
+    U[0.8, 1.2], a ~ U[0.5, 4] AU, e ~ Rayleigh(0.01), i ~ Rayleigh(0.005 rad),
+    remaining angles uniform.  Physical radii from a density of 3 g/cm^3 go in
+    meta["radius"]; the configuration integrates with dt 0.002 yr, r_c = 3 Hill
+    radii and BS tolerance 1e-11 (meta)."""
+    rng = _rng(7, seed_key)
+    m = 3.0e-7 * rng.uniform(0.8, 1.2, n)
+    a = rng.uniform(0.5, 4.0, n)
+    e = rng.rayleigh(0.01, n)
+    inc = rng.rayleigh(0.005, n)
+    ang = rng.uniform(0.0, 2 * np.pi, (3, n))
+    x, v = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], e, a)
+    return mint.State(np.array([0, 0], np.int64), m[None], x[None], v[None], np.zeros((0, 3)), np.zeros((0, 3)),
+                      meta=dict(workload="embryos", dt=0.002, n_hill=3.0, bs_tol=1e-11,
+                                radius=embryo_radii(m)[None]))

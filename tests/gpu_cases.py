@@ -32,3 +32,60 @@ def run_gpu(st, nsteps, dt, form, guess, stats=1, **kw):
         g.upload(st)
         g.step(nsteps)
         return g.download(), g.status(), g.stats()
+
+
+def embryo_ring(n=64, seed=0, a_lo=1.0, a_hi=1.02, mass=3.0e-7):
+    """n embryos packed into a narrow ring (a in [a_lo, a_hi] AU, e and i
+    Rayleigh 0.01 / 0.005): dense enough that several encounter groups, some
+    of three or more bodies, form within tens of steps at dt = 0.002."""
+    rng = np.random.default_rng([20261203, 8, seed])
+    m = mass * rng.uniform(0.8, 1.2, n)
+    a = rng.uniform(a_lo, a_hi, n)
+    e = rng.rayleigh(0.01, n)
+    inc = rng.rayleigh(0.005, n)
+    ang = rng.uniform(0.0, 2 * np.pi, (3, n))
+    x, v = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], e, a)
+    return mint.State(np.array([0, 0], np.int64), m[None], x[None], v[None], np.zeros((0, 3)), np.zeros((0, 3)),
+                      meta=dict(workload="embryo_ring", dt=0.002, radius=W.embryo_radii(m)[None]))
+
+
+def embryo_cluster(n=8, seed=0, span=0.06, a_lo=1.0, a_hi=1.004, mass=3.0e-7):
+    """n embryos within `span` radians of orbital longitude and a in
+    [a_lo, a_hi]: they start in and out of each other's changeover zones, so
+    groups form, merge and split over a few hundred steps."""
+    rng = np.random.default_rng([20261203, 9, seed])
+    m = mass * rng.uniform(0.8, 1.2, n)
+    a = rng.uniform(a_lo, a_hi, n)
+    e = rng.rayleigh(0.002, n)
+    inc = rng.rayleigh(0.001, n)
+    z = np.zeros(n)
+    x, v = mint.elements_to_cartesian(1.0, inc, z, z, rng.uniform(0.0, span, n), e, a)
+    return mint.State(np.array([0, 0], np.int64), m[None], x[None], v[None], np.zeros((0, 3)), np.zeros((0, 3)),
+                      meta=dict(workload="embryo_cluster", dt=0.002, radius=W.embryo_radii(m)[None]))
+
+
+def coorbital_system(n_tp=64, seed=5):
+    """Sweep system 0 with Neptune (body 4) replaced by a pair of 1e-5 Msun
+    co-orbitals 0.02 rad apart at its distance, which start inside their
+    changeover radius and stay near it (a massive encounter group beside three
+    free bodies), plus TPs around the pair."""
+    st = W.p9_sweep(systems=[0], tp_scale=1 / 64)
+    m = np.concatenate([st.m[0, :4], [1e-5, 1.2e-5]])
+    a = 30.1
+    phase = np.random.default_rng(seed).uniform(0, 2 * np.pi)
+    els = [np.array([0.001, 0.002]), np.array([0.3, 0.3]), np.array([0.1, 0.1]),
+           np.array([phase, phase + 0.02]), np.array([0.001, 0.002]), np.array([a, a])]
+    x2, v2 = mint.elements_to_cartesian(1.0, *els)
+    mx = np.concatenate([st.mx[0, :4], x2])
+    mv = np.concatenate([st.mv[0, :4], v2])
+    rc = mint.hill_radii(1.0, 3.0, m, mx, mv)
+    rng = np.random.default_rng(seed + 1)
+    tx, tv = [], []
+    for i in range(n_tp):
+        j = 4 + (i % 2)
+        d = rng.normal(size=3)
+        d *= rc[j] * rng.uniform(0.2, 2.0) / np.linalg.norm(d)
+        tx.append(mx[j] + d)
+        tv.append(mv[j] + rng.normal(size=3) * 0.01 * np.linalg.norm(mv[j]))
+    return mint.State(np.array([0, n_tp], np.int64), m[None], mx[None], mv[None], np.array(tx), np.array(tv),
+                      meta=dict(workload="coorbital", dt=0.5))

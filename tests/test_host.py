@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 25
     missing = [s for s in syms if not hasattr(L, s)]
     assert not missing, missing
-    assert L.mint_abi_version() == 1
+    assert L.mint_abi_version() == 2
 
 
 def test_python_mirror_binds_every_declared_symbol():
