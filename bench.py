@@ -2,6 +2,7 @@
 sweep (dt = 0.5 yr), one MI355X, through libmint's C ABI.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--form learned|batched|flat|plain]
+  python bench.py --workload embryos [--steps K]     (the 1000-embryo all-massive system)
 
 A "step" is one Mercurius step of every system of the sweep (about 5.6 M test
 particles, five massive bodies each).  The state is uploaded to HBM before the
@@ -131,10 +132,101 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+EMBRYO_METRIC = "massive body-steps/s, 1000-embryo system (all massive, O(N^2) kicks), dt 0.002"
+
+
+def flops_per_kick_pass(n: int) -> float:
+    """Algorithmic FLOP of one kick-acceleration pass over n massive bodies:
+    n (n - 1) ordered pairs at 28 FLOP each (the per-pair kick figure of the
+    test-particle count above); one pass per step, both half-kicks use it."""
+    return 28.0 * n * (n - 1)
+
+
+def bench_embryos(args):
+    """The all-massive embryo configuration on the many-body engine: K steps
+    timed after W warm-up steps, kernel timing from a separate profiling pass,
+    the oracle's single-thread rate on the same system as the CPU baseline."""
+    st = W.embryos(1000)
+    rad = st.meta["radius"]
+    n = st.n_massive
+    kw = dict(dt=0.002, n_hill=3.0, bs_tol=1e-11)
+    g = mint.Mint(**kw)
+    g.upload(st)
+    g.set_radii(rad)
+    if args.warmup > 0:
+        g.step(args.warmup)
+    g.upload(st)
+    g.set_radii(rad)
+    E0 = g.energy()
+    g.reset_counters()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    wall = time.perf_counter() - t0
+    ev_s = g.timing()["ms_total"] / 1e3
+    E1 = g.energy()
+    status = int(g.status()[0])
+    fin = g.download()
+    g.close()
+    value = n * args.steps / ev_s
+    log(f"bench: embryos: {args.steps} steps in {ev_s * 1e3:.1f} ms -> {value:.4e} body-steps/s")
+    # profiling pass: per-kernel HIP-event timing and work counters (no graph)
+    gp = mint.Mint(collect_stats=3, **kw)
+    gp.upload(st)
+    gp.set_radii(rad)
+    nprof = min(args.steps, 256)
+    gp.reset_counters()
+    gp.step(nprof)
+    t = gp.timing()
+    s = gp.stats()
+    gp.close()
+    ms = {k: t[f"ms_mb_{k}"] / max(t[f"n_mb_{k}"], 1) for k in ("open", "drift", "close")}
+    F = flops_per_kick_pass(n)
+    achieved = F / (ms["close"] / 1e3)
+    roof = {"bound": "fp64", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "mb_close_kernel",
+            "flop_per_launch": F, "avg_launch_ms": ms}
+    base = None
+    check = {}
+    if not args.no_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # test infrastructure: checker and CPU baseline leg only
+
+        nb = 1000
+        t0 = time.perf_counter()
+        ref, _, _ = O.run(st, nb, 0.002, bs_tol=1e-11, radii=rad)
+        dt = time.perf_counter() - t0
+        base = dict(value=n * nb / dt, unit="body-steps/s", cores=1, kind="port",
+                    sample=f"the same 1000-embryo system, {nb} steps, oracle single thread, {dt:.1f} s")
+        gc = mint.Mint(**kw)
+        gc.upload(st)
+        gc.set_radii(rad)
+        gc.step(nb)
+        out = gc.download()
+        gc.close()
+        check = {"first_steps_bitwise_equal_to_oracle": bool(np.array_equal(out.mx, ref["mx"]) and
+                                                              np.array_equal(out.mv, ref["mv"])),
+                 "steps_compared": nb}
+        log(f"bench: cpu baseline {base}")
+    out = {
+        "metric": EMBRYO_METRIC, "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ev_s * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "embryos_1000", "bodies": n, "dt_yr": 0.002, "n_hill": 3.0, "bs_tol": 1e-11,
+                   "engine": "many-body"},
+        "roofline": roof, "cpu_baseline": base,
+        "energy": {"max_rel_dE_over_run": float(abs((E1[0] - E0[0]) / E0[0])), "status_bits": status, **check},
+        "profile": {"steps": nprof, "stats": s, "timing": t,
+                    "groups_per_step": s["massive_groups"] / nprof,
+                    "bodies_in_groups_per_step": s["massive_group_bodies"] / nprof},
+        "host_wall_s": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=None, help="default 2000 (sweep), 10000 (embryos)")
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--form", default="learned", choices=["learned", "batched", "flat", "plain"])
     ap.add_argument("--systems", type=int, default=4096)
@@ -144,6 +236,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "p9-integrator_amd", "autotune.cache"))
     ap.add_argument("--shape", default=None, help="block,tps_per_lane,window: skip the autotuner")
+    ap.add_argument("--workload", default="p9_sweep", choices=["p9_sweep", "embryos"],
+                    help="embryos: the 1000-embryo all-massive system (default steps 10000)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -154,6 +248,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (systems would shard "
                                    "across GPUs with no exchange, see the sharding paragraph of DESIGN.md)"}))
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 10000 if args.workload == "embryos" else 2000
+    if args.workload == "embryos":
+        bench_embryos(args)
+        return
 
     log(f"bench: generating the {args.systems}-system sweep")
     st = W.p9_sweep(S=4096, systems=np.arange(args.systems))

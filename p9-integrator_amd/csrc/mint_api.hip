@@ -70,7 +70,7 @@ void Ctx::free_device() {
   mb_graph = nullptr;
   mb_graph_steps = 0;
   d_bp = d_mxb = d_mvb = d_my = nullptr;
-  d_mflag = d_medge = d_gstart = d_gmem = nullptr;
+  d_mflag = d_medge = nullptr;
   d_mctl = nullptr;
   many = false;
   rec_steps_alloc = 0;

@@ -74,8 +74,7 @@ struct Ctx {
   double *d_my = nullptr;           // post-jump positions (drift start) [S][NM][3]
   int *d_mflag = nullptr;           // [S][NM] body is in an encounter pair this step
   int *d_medge = nullptr;           // [S][MB_EMAX][2] encounter pairs
-  unsigned int *d_mctl = nullptr;   // [S][4] pairs, blocks done, groups, grouped bodies
-  int *d_gstart = nullptr, *d_gmem = nullptr;  // [S][NM+1] group starts, [S][NM] members
+  unsigned int *d_mctl = nullptr;   // [S][4]: [0] pairs in the list this step
   hipGraphExec_t mb_graph = nullptr;
   long long mb_graph_steps = 0;
 
@@ -124,7 +123,7 @@ struct KTimer {
 
 // many-body engine (mint_mb.hip): systems of more than MAX_NM massive bodies
 constexpr int MB_MAXN = 2048;   // massive bodies per system
-constexpr int MB_EMAX = 2048;   // encounter pairs per system and step
+constexpr int MB_EMAX = 1024;   // encounter pairs per system and step
 constexpr int MB_GMAX = 64;     // bodies per encounter group (one wavefront)
 int mb_alloc(Ctx *c);       // engine buffers for the uploaded shape
 int mb_reset(Ctx *c);       // body parameters from m, rc, rad; clear flags and counters

@@ -8,12 +8,13 @@
 //
 //   mb_open_kernel   momentum after the opening kick, jump 1, and the exact
 //                    pair check of every pair (j < k) against its changeover
-//                    radius; pairs inside are appended to an edge list.  The
-//                    last workgroup to finish turns the edges into encounter
-//                    groups (connected components, listed by smallest member).
-//   mb_drift_kernel  Kepler drift of every body in no group (one lane each);
-//                    Bulirsch-Stoer drift of each group (one wavefront each,
-//                    one member per lane, positions exchanged through LDS).
+//                    radius; pairs inside are appended to the step's pair list.
+//   mb_drift_kernel  Kepler drift of every body in no pair (one lane each);
+//                    meanwhile each group workgroup turns the pair list into
+//                    encounter groups (connected components, listed by
+//                    smallest member) in its LDS and drifts its share of them
+//                    by Bulirsch-Stoer (one wavefront per group, one member
+//                    per lane, positions exchanged through LDS).
 //   mb_close_kernel  momentum after the drift, jump 2, O(N^2) kick
 //                    accelerations and the closing kick (fused with the next
 //                    step's opening kick, which sees the same positions).
@@ -42,10 +43,12 @@ struct MBShape {
   int N;    // bodies per system
   int nch;  // 16-body chunks
   int TI;   // bodies per workgroup (open/close kernels)
-  int CS;   // chunk slots per workgroup; TI * CS = MB_BLOCK
+  int CS;   // chunk slots per workgroup; TI * CS = MB_BLOCK; a window of CS
+            // chunks of partner data is staged in LDS at a time
 };
 
 constexpr int MB_BLOCK = 256;
+constexpr int MB_MAXCS = 64;   // chunk slots: a staged window holds up to 64 x 16 bodies
 constexpr int MB_DRIFT_GROUP_BLOCKS = 32;  // workgroups that walk the group list
 constexpr int MB_GRAPH_STEPS = 64;
 
@@ -54,7 +57,7 @@ MBShape mb_shape(int N) {
   s.N = N;
   s.nch = (N + MB_CHUNK - 1) / MB_CHUNK;
   int cs = 1;
-  while (cs < s.nch && cs < MB_BLOCK) cs <<= 1;
+  while (cs < s.nch && cs < MB_MAXCS) cs <<= 1;
   s.CS = cs;
   s.TI = MB_BLOCK / cs;
   return s;
@@ -70,18 +73,61 @@ __device__ __forceinline__ BP bp_of(const double *bp_all, int s, int N) {
   return BP{b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
 }
 
+// Left fold p[0] + p[1] + ... + p[n-1] of n three-vectors stored [n][3] in
+// LDS (loads issued eight at a time ahead of the dependent adds).
+__device__ __forceinline__ void fold3(const double *p, int n, int stride, double &x, double &y, double &z) {
+  x = p[0];
+  y = p[1];
+  z = p[2];
+  int c = 1;
+  for (; c + 8 <= n; c += 8) {
+    double tx[8], ty[8], tz[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double *q = p + (size_t)(c + u) * stride;
+      tx[u] = q[0];
+      ty[u] = q[1];
+      tz[u] = q[2];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      x = x + tx[u];
+      y = y + ty[u];
+      z = z + tz[u];
+    }
+  }
+  for (; c < n; ++c) {
+    const double *q = p + (size_t)c * stride;
+    x = x + q[0];
+    y = y + q[1];
+    z = z + q[2];
+  }
+}
+
 // Q = (sum_j m_j v_j)/M by the whole workgroup: chunk sums (from zero, index
 // order) in part[], then added in chunk order by one thread.
 __device__ void wg_momentum(const double *__restrict__ V, const double *__restrict__ m, int N,
                             int nch, double M, double *part, double *Q) {
   for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+    const int k0 = c * MB_CHUNK;
+    double mm[MB_CHUNK], vx[MB_CHUNK], vy[MB_CHUNK], vz[MB_CHUNK];
+#pragma unroll
+    for (int q = 0; q < MB_CHUNK; ++q) {
+      const int k = k0 + q;
+      const bool in = k < N;
+      mm[q] = in ? m[k] : 0.0;
+      vx[q] = in ? V[3 * k + 0] : 0.0;
+      vy[q] = in ? V[3 * k + 1] : 0.0;
+      vz[q] = in ? V[3 * k + 2] : 0.0;
+    }
     double px = 0.0, py = 0.0, pz = 0.0;
-    const int k1 = min(N, (c + 1) * MB_CHUNK);
-    for (int k = c * MB_CHUNK; k < k1; ++k) {
-      const double mk = m[k];
-      px = fma(mk, V[3 * k + 0], px);
-      py = fma(mk, V[3 * k + 1], py);
-      pz = fma(mk, V[3 * k + 2], pz);
+#pragma unroll
+    for (int q = 0; q < MB_CHUNK; ++q) {
+      if (k0 + q < N) {
+        px = fma(mm[q], vx[q], px);
+        py = fma(mm[q], vy[q], py);
+        pz = fma(mm[q], vz[q], pz);
+      }
     }
     part[3 * c + 0] = px;
     part[3 * c + 1] = py;
@@ -89,12 +135,8 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double qx = part[0], qy = part[1], qz = part[2];
-    for (int c = 1; c < nch; ++c) {
-      qx = qx + part[3 * c + 0];
-      qy = qy + part[3 * c + 1];
-      qz = qz + part[3 * c + 2];
-    }
+    double qx, qy, qz;
+    fold3(part, nch, 3, qx, qy, qz);
     Q[0] = qx / M;
     Q[1] = qy / M;
     Q[2] = qz / M;
@@ -112,17 +154,23 @@ __global__ void __launch_bounds__(MB_BLOCK)
     mb_close_kernel(MBShape sh, const double *__restrict__ bp_all, double *__restrict__ xa_all,
                     double *__restrict__ va_all, const double *__restrict__ xb_all,
                     const double *__restrict__ vb_all, int kicks, int last,
-                    unsigned int *__restrict__ status, StepConst sc, double M) {
+                    unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
+                    StepConst sc, double M) {
   extern __shared__ double lds[];
   const int N = sh.N, s = blockIdx.y;
+  // the pair list of this step has been consumed by the drift: empty it
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_all[4 * s] = 0u;
   const BP bp = bp_of(bp_all, s, N);
   double *XA = xa_all + (size_t)s * N * 3;
   double *VA = va_all + (size_t)s * N * 3;
   const double *XB = xb_all + (size_t)s * N * 3;
   const double *VB = vb_all + (size_t)s * N * 3;
-  double *part = lds;                            // [nch][TI][3]
-  double *qpart = lds + (size_t)sh.nch * sh.TI * 3;  // [nch][3]
-  double *Qs = qpart + 3 * sh.nch;               // [3]
+  const int WB = sh.CS * MB_CHUNK;                   // bodies per staged window
+  double *part = lds;                                // [nch][TI][3]
+  double *qpart = part + (size_t)sh.nch * sh.TI * 3;  // [nch][3]
+  double *Qs = qpart + 3 * sh.nch;                   // [3]
+  double *sx = Qs + 3;                               // [WB][3] partner positions
+  double *sm = sx + 3 * WB, *slo = sm + WB, *swv = slo + WB;
   const double *X = INIT ? XA : XB;
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (!INIT) {
@@ -147,7 +195,27 @@ __global__ void __launch_bounds__(MB_BLOCK)
     loi = bp.lo[i];
     wvi = bp.winv[i];
   }
-  for (int c = cs; c < sh.nch; c += sh.CS) {
+  for (int w0 = 0; w0 < N; w0 += WB) {
+    __syncthreads();
+    const int nw = min(WB, N - w0);
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+      const int kk = w0 + k;
+      double xk = X[3 * kk + 0], yk = X[3 * kk + 1], zk = X[3 * kk + 2];
+      if (!INIT) {
+        xk = fma(sc.hh, q0, xk);
+        yk = fma(sc.hh, q1, yk);
+        zk = fma(sc.hh, q2, zk);
+      }
+      sx[3 * k + 0] = xk;
+      sx[3 * k + 1] = yk;
+      sx[3 * k + 2] = zk;
+      sm[k] = bp.m[kk];
+      slo[k] = bp.lo[kk];
+      swv[k] = bp.winv[kk];
+    }
+    __syncthreads();
+    const int c = w0 / MB_CHUNK + cs;
+    if (c >= sh.nch) continue;
     double px = 0.0, py = 0.0, pz = 0.0;
     if (act) {
 #pragma unroll
@@ -156,25 +224,19 @@ __global__ void __launch_bounds__(MB_BLOCK)
 #pragma unroll
         for (int q = 0; q < MB_DIVG; ++q) {
           const int k = c * MB_CHUNK + hg * MB_DIVG + q;
+          const int kl = k - w0;
           dx[q] = 0.0;
           dy[q] = 0.0;
           dz[q] = 0.0;
           num[q] = 0.0;
           den[q] = 1.0;
           if (k < N && k != i) {
-            double xk = X[3 * k + 0], yk = X[3 * k + 1], zk = X[3 * k + 2];
-            if (!INIT) {
-              xk = fma(sc.hh, q0, xk);
-              yk = fma(sc.hh, q1, yk);
-              zk = fma(sc.hh, q2, zk);
-            }
-            dx[q] = xk - xi;
-            dy[q] = yk - yi;
-            dz[q] = zk - zi;
+            dx[q] = sx[3 * kl + 0] - xi;
+            dy[q] = sx[3 * kl + 1] - yi;
+            dz[q] = sx[3 * kl + 2] - zi;
             const double r2 = dot3(dx[q], dy[q], dz[q], dx[q], dy[q], dz[q]);
-            const double lok = bp.lo[k], wvk = bp.winv[k];
-            kick_nd(r2, sc.eps2, bp.m[k], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, num[q],
-                    den[q]);
+            const double lok = slo[kl], wvk = swv[kl];
+            kick_nd(r2, sc.eps2, sm[kl], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, num[q], den[q]);
           }
         }
         batched_div<MB_DIVG>(num, den, w);
@@ -193,13 +255,8 @@ __global__ void __launch_bounds__(MB_BLOCK)
   }
   __syncthreads();
   if (cs == 0 && act) {
-    double ax = part[3 * ti + 0], ay = part[3 * ti + 1], az = part[3 * ti + 2];
-    for (int c = 1; c < sh.nch; ++c) {
-      const double *pp = part + ((size_t)c * sh.TI + ti) * 3;
-      ax = ax + pp[0];
-      ay = ay + pp[1];
-      az = az + pp[2];
-    }
+    double ax, ay, az;
+    fold3(part + 3 * ti, sh.nch, 3 * sh.TI, ax, ay, az);
     const double *V = INIT ? VA : VB;
     double vx = V[3 * i + 0], vy = V[3 * i + 1], vz = V[3 * i + 2];
     vx = fma(sc.hh, ax, vx);
@@ -225,145 +282,133 @@ __global__ void __launch_bounds__(MB_BLOCK)
 }
 
 // ---------------------------------------------------------------------------
-// Encounter groups from the edge list, by the last workgroup of the open
-// kernel.  Labels: min-label propagation over the edges until no label
-// changes, so every member ends with its group's smallest index.  Then one
-// block-wide exclusive scan over the bodies of (flagged, root, size at root)
-// gives each flagged body its slot in the compact list, each root its group
-// number and the group's start in the member list; members keep index order.
-// LDS: Lb[N], Fb[N], Cb[N], E[2 MB_EMAX], scr[64] (ints).
-__device__ void build_groups(int N, unsigned int ne, const int *__restrict__ edge,
-                             unsigned int *ctl, int *__restrict__ gstart, int *__restrict__ gmem,
-                             int *Lb, int *Fb, int *Cb, int *E, int *scr,
-                             unsigned int *status_s, unsigned long long *stats) {
-  const int tid = threadIdx.x, bd = blockDim.x;
-  for (int i = tid; i < N; i += bd) {
-    Lb[i] = i;
-    Fb[i] = 0;
-    Cb[i] = 0;
+// Encounter groups from the pair list, built in LDS by one wavefront (the
+// workgroup of a group worker).  The bodies in pairs are marked in a bitmap of
+// the system's N <= 2048 bodies (64 words, one per lane); a scan of the word
+// popcounts ranks them, which gives the sorted compact list fl[F] and each
+// pair's ends as compact indices.  Min-label propagation over the pairs then
+// runs until no label changes, so every body carries the compact index of its
+// group's smallest member (the root).  Groups are numbered by root; a group
+// starts after the members of the groups with smaller roots, and a member's
+// place in its group is the number of members before it (all of which lie
+// between the root and itself in the compact order).  Outputs gst[G+1] and
+// gm[F] (body indices); returns G, and sets big when a group has more than
+// MB_GMAX members.  LDS (ints): bm[64], pre[64], and fl, L, ce, cnt, gst, gm
+// of 2 MB_EMAX each (gst one more).
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+__device__ int build_groups(int N, int ne, const int *__restrict__ edge, unsigned int *bm, int *pre,
+                            int *fl, int *L, int *ce, int *cnt, int *gst, int *gm, bool &big) {
+  const int lane = threadIdx.x;  // one wavefront
+  const int n2 = 2 * ne, nwd = (N + 31) >> 5;
+  bm[lane] = 0u;
+  __syncthreads();
+  for (int p = lane; p < n2; p += 64) {
+    const int b = edge[p];
+    ce[p] = b;
+    atomicOr(&bm[b >> 5], 1u << (b & 31));
   }
   __syncthreads();
-  for (int e = tid; e < (int)ne; e += bd) {
-    const int a = edge[2 * e], b = edge[2 * e + 1];
-    E[2 * e] = a;
-    E[2 * e + 1] = b;
-    Fb[a] = 1;
-    Fb[b] = 1;
+  const unsigned int word = lane < nwd ? bm[lane] : 0u;
+  const int pc = __popc(word);
+  const int inc = wave_incl_scan(pc, lane);
+  pre[lane] = inc - pc;
+  const int F = __shfl(inc, 63, 64);
+  {
+    unsigned int w = word;
+    int r = inc - pc;
+    while (w) {
+      const int b = __ffs(w) - 1;
+      fl[r++] = lane * 32 + b;
+      w &= w - 1u;
+    }
+  }
+  __syncthreads();
+  for (int p = lane; p < n2; p += 64) {
+    const int b = ce[p];
+    ce[p] = pre[b >> 5] + __popc(bm[b >> 5] & ((1u << (b & 31)) - 1u));
+  }
+  for (int c = lane; c < F; c += 64) {
+    L[c] = c;
+    cnt[c] = 0;
   }
   __syncthreads();
   for (;;) {
-    if (tid == 0) scr[0] = 0;
-    __syncthreads();
-    for (int e = tid; e < (int)ne; e += bd) {
-      const int a = E[2 * e], b = E[2 * e + 1];
-      const int la = Lb[a], lb = Lb[b];
+    int changed = 0;
+    for (int e = lane; e < ne; e += 64) {
+      const int a = ce[2 * e], b = ce[2 * e + 1];
+      const int la = L[a], lb = L[b];
       if (la < lb) {
-        atomicMin(&Lb[b], la);
-        scr[0] = 1;
+        atomicMin(&L[b], la);
+        changed = 1;
       } else if (lb < la) {
-        atomicMin(&Lb[a], lb);
-        scr[0] = 1;
+        atomicMin(&L[a], lb);
+        changed = 1;
       }
     }
     __syncthreads();
-    const int changed = scr[0];
-    __syncthreads();
-    if (!changed) break;
+    if (!__any(changed)) break;
   }
-  for (int i = tid; i < N; i += bd)
-    if (Fb[i]) atomicAdd(&Cb[Lb[i]], 1);
+  for (int c = lane; c < F; c += 64) atomicAdd(&cnt[L[c]], 1);
   __syncthreads();
-  // block-wide exclusive scan of (flagged, root, size at root), PER bodies a thread
-  const int PER = (N + bd - 1) / bd;
-  const int e0 = tid * PER;
-  int sf = 0, sr = 0, sz = 0;
+  // scan over the compact bodies of (root, size at root), PER a lane
+  const int PER = (F + 63) >> 6;
+  const int c0 = lane * PER;
+  int sr = 0, sz = 0;
+  bool mybig = false;
   for (int u = 0; u < PER; ++u) {
-    const int e = e0 + u;
-    if (e < N && Fb[e]) {
-      sf += 1;
-      if (Lb[e] == e) {
-        sr += 1;
-        sz += Cb[e];
-      }
+    const int c = c0 + u;
+    if (c < F && L[c] == c) {
+      sr += 1;
+      sz += cnt[c];
+      mybig = mybig || cnt[c] > MB_GMAX;
     }
   }
-  const int lane = tid & 63, wv = tid >> 6, nw = bd >> 6;
-  int inf = sf, inr = sr, inz = sz;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int tf = __shfl_up(inf, off, 64), tr = __shfl_up(inr, off, 64), tz = __shfl_up(inz, off, 64);
-    if (lane >= off) {
-      inf += tf;
-      inr += tr;
-      inz += tz;
-    }
-  }
-  if (lane == 63) {
-    scr[1 + 3 * wv] = inf;
-    scr[2 + 3 * wv] = inr;
-    scr[3 + 3 * wv] = inz;
-  }
-  __syncthreads();
-  int bf = 0, br = 0, bz = 0, tf = 0, tr = 0;
-  for (int w = 0; w < nw; ++w) {
-    if (w < wv) {
-      bf += scr[1 + 3 * w];
-      br += scr[2 + 3 * w];
-      bz += scr[3 + 3 * w];
-    }
-    tf += scr[1 + 3 * w];
-    tr += scr[2 + 3 * w];
-  }
-  int xf = bf + inf - sf, xr = br + inr - sr, xz = bz + inz - sz;
-  int *fl = E;  // the edges are no longer needed: compact flagged list
-  bool big = false;
+  const int ir = wave_incl_scan(sr, lane), iz = wave_incl_scan(sz, lane);
+  const int G = __shfl(ir, 63, 64);
+  int xr = ir - sr, xz = iz - sz;
   __syncthreads();
   for (int u = 0; u < PER; ++u) {
-    const int e = e0 + u;
-    if (e < N && Fb[e]) {
-      fl[xf++] = e;
-      if (Lb[e] == e) {
-        const int size = Cb[e];
-        big = big || size > MB_GMAX;
-        gstart[xr++] = xz;
-        Cb[e] = xz;  // group start, read by the members below
-        xz += size;
-      }
+    const int c = c0 + u;
+    if (c < F && L[c] == c) {
+      const int size = cnt[c];
+      gst[xr++] = xz;
+      cnt[c] = xz;  // group start, read by the members below
+      xz += size;
     }
   }
   __syncthreads();
-  for (int e = tid; e < N; e += bd) {
-    if (!Fb[e]) continue;
-    const int r = Lb[e];
+  for (int c = lane; c < F; c += 64) {
+    const int r = L[c];
     int rank = 0;
-    for (int p = 0; fl[p] != e; ++p) rank += (Lb[fl[p]] == r);
-    gmem[Cb[r] + rank] = e;
+    for (int q = r; q < c; ++q) rank += (L[q] == r);
+    gm[cnt[r] + rank] = fl[c];
   }
-  if (big) atomicOr(status_s, ST_LIMIT);
-  if (tid == 0) {
-    gstart[tr] = tf;
-    ctl[2] = (unsigned int)tr;
-    ctl[3] = (unsigned int)tf;
-    if (tr > 0) atomicOr(status_s, ST_MASSIVE_ENC);
-    if (stats) {
-      atomicAdd(stats + S_MGROUPS, (unsigned long long)tr);
-      atomicAdd(stats + S_MGBODIES, (unsigned long long)tf);
-    }
-  }
+  if (lane == 0) gst[G] = F;
+  big = __any(mybig);
+  __syncthreads();
+  return G;
 }
 
 // Momentum after the opening kick, jump 1 (stored to Y by the body's owner)
 // and the exact pair check (close_encounter_p, src/sim.cuh:589-613) of every
-// pair j < k against r_c^2 of the pair.  Also counts pairs closer than the
-// sum of their physical radii.
+// pair j < k against r_c^2 of the pair; pairs inside are appended to the pair
+// list and both bodies flagged.  Also counts pairs closer than the sum of
+// their physical radii.
 __global__ void __launch_bounds__(MB_BLOCK)
     mb_open_kernel(MBShape sh, const double *__restrict__ bp_all, const double *__restrict__ xa_all,
                    const double *__restrict__ va_all, double *__restrict__ y_all,
                    int *__restrict__ flag_all, int *__restrict__ edge_all,
-                   unsigned int *__restrict__ ctl_all, int *__restrict__ gstart_all,
-                   int *__restrict__ gmem_all, unsigned int *__restrict__ status,
+                   unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
                    unsigned long long *__restrict__ stats, StepConst sc, double M) {
   extern __shared__ double lds[];
-  __shared__ int is_last;
   const int N = sh.N, s = blockIdx.y;
   const BP bp = bp_of(bp_all, s, N);
   const double *XA = xa_all + (size_t)s * N * 3;
@@ -372,119 +417,104 @@ __global__ void __launch_bounds__(MB_BLOCK)
   int *flag = flag_all + (size_t)s * N;
   int *edge = edge_all + (size_t)s * MB_EMAX * 2;
   unsigned int *ctl = ctl_all + 4 * s;
+  const int WB = sh.CS * MB_CHUNK;
   double *qpart = lds;
   double *Qs = lds + 3 * sh.nch;
+  double *sx = Qs + 3;                 // [WB][3] partner positions after jump 1
+  double *src2 = sx + 3 * WB, *srad = src2 + WB;
   wg_momentum(VA, bp.m, N, sh.nch, M, qpart, Qs);
   const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
   const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
   const int i = (int)blockIdx.x * sh.TI + ti;
   const bool act = i < N;
   unsigned int ov = 0;
+  double xi = 0.0, yi = 0.0, zi = 0.0, rc2i = 0.0, radi = 0.0;
   if (act) {
-    const double xi = fma(sc.hh, q0, XA[3 * i + 0]);
-    const double yi = fma(sc.hh, q1, XA[3 * i + 1]);
-    const double zi = fma(sc.hh, q2, XA[3 * i + 2]);
+    xi = fma(sc.hh, q0, XA[3 * i + 0]);
+    yi = fma(sc.hh, q1, XA[3 * i + 1]);
+    zi = fma(sc.hh, q2, XA[3 * i + 2]);
     if (cs == 0) {
       Y[3 * i + 0] = xi;
       Y[3 * i + 1] = yi;
       Y[3 * i + 2] = zi;
     }
-    const double rc2i = bp.rc2[i], radi = bp.rad[i];
-    for (int c = cs; c < sh.nch; c += sh.CS) {
-      const int k1 = min(N, (c + 1) * MB_CHUNK);
-      for (int k = max(c * MB_CHUNK, i + 1); k < k1; ++k) {
-        const double dx = fma(sc.hh, q0, XA[3 * k + 0]) - xi;
-        const double dy = fma(sc.hh, q1, XA[3 * k + 1]) - yi;
-        const double dz = fma(sc.hh, q2, XA[3 * k + 2]) - zi;
-        const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-        const double sr = radi + bp.rad[k];
-        if (r2 < sr * sr) ++ov;
-        const double rc2k = bp.rc2[k];
-        if (r2 < (rc2i > rc2k ? rc2i : rc2k)) {
-          const unsigned int slot = atomicAdd(ctl + 0, 1u);
-          if (slot < (unsigned int)MB_EMAX) {
-            edge[2 * slot] = i;
-            edge[2 * slot + 1] = k;
-          }
-          flag[i] = 1;
-          flag[k] = 1;
+    rc2i = bp.rc2[i];
+    radi = bp.rad[i];
+  }
+  for (int w0 = 0; w0 < N; w0 += WB) {
+    __syncthreads();
+    const int nw = min(WB, N - w0);
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+      const int kk = w0 + k;
+      sx[3 * k + 0] = fma(sc.hh, q0, XA[3 * kk + 0]);
+      sx[3 * k + 1] = fma(sc.hh, q1, XA[3 * kk + 1]);
+      sx[3 * k + 2] = fma(sc.hh, q2, XA[3 * kk + 2]);
+      src2[k] = bp.rc2[kk];
+      srad[k] = bp.rad[kk];
+    }
+    __syncthreads();
+    const int c = w0 / MB_CHUNK + cs;
+    if (!act || c >= sh.nch || (c + 1) * MB_CHUNK <= i + 1) continue;
+#pragma unroll
+    for (int q = 0; q < MB_CHUNK; ++q) {
+      const int k = c * MB_CHUNK + q;
+      if (k <= i || k >= N) continue;
+      const int kl = k - w0;
+      const double dx = sx[3 * kl + 0] - xi;
+      const double dy = sx[3 * kl + 1] - yi;
+      const double dz = sx[3 * kl + 2] - zi;
+      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double sr = radi + srad[kl];
+      if (r2 < sr * sr) ++ov;
+      const double rc2k = src2[kl];
+      if (r2 < (rc2i > rc2k ? rc2i : rc2k)) {
+        const unsigned int slot = atomicAdd(ctl + 0, 1u);
+        if (slot < (unsigned int)MB_EMAX) {
+          edge[2 * slot] = i;
+          edge[2 * slot + 1] = k;
         }
+        flag[i] = 1;
+        flag[k] = 1;
       }
     }
   }
   if (ov) atomicOr(status + s, ST_OVERLAP);
   if (stats) wave_add(stats + S_OVERLAP, ov);
-  // the last workgroup of this system builds the groups
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = (atomicAdd(ctl + 1, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const unsigned int ne = atomicAdd(ctl + 0, 0u);
-  if (ne == 0 || ne > (unsigned int)MB_EMAX) {
-    if (threadIdx.x == 0) {
-      if (ne > (unsigned int)MB_EMAX) atomicOr(status + s, ST_LIMIT);
-      ctl[2] = 0u;
-      ctl[3] = 0u;
-      ctl[0] = 0u;
-      ctl[1] = 0u;
-    }
-    return;
-  }
-  int *ib = (int *)(lds + 3 * sh.nch + 3);
-  build_groups(N, ne, edge, ctl, gstart_all + (size_t)s * (N + 1), gmem_all + (size_t)s * N, ib,
-               ib + N, ib + 2 * N, ib + 3 * N, ib + 3 * N + 2 * MB_EMAX, status + s, stats);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ctl[0] = 0u;
-    ctl[1] = 0u;
-  }
 }
 
 // ---------------------------------------------------------------------------
 // Bulirsch-Stoer drift of one encounter group by one wavefront: lane a holds
-// member a's state (6 doubles) and tableau; each right-hand side publishes the
-// members' positions in LDS.  Same operations, per member and component, as
-// the single-lane group BS; the error is the max over members, which does not
+// member a's state (6 doubles), its mass and changeover fields, and its
+// tableau; each right-hand side reads the other members' positions and fields
+// by wavefront shuffles.  Same operations, per member and component, as the
+// single-lane group BS; the error is the max over members, which does not
 // depend on the order it is taken in.
 __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu, double eps2,
-                                  double tol, const double *pm, const double *plo,
-                                  const double *pwv, double *pos, unsigned int &nrhs) {
+                                  double tol, double mj, double loj, double wvj, unsigned int &nrhs) {
   const bool act = a < gs;
   auto rhs = [&](const double *z, double *f) {
-    pos[3 * a + 0] = z[0];
-    pos[3 * a + 1] = z[1];
-    pos[3 * a + 2] = z[2];
-    __syncthreads();
+    const double x = z[0], yy = z[1], zz = z[2];
     f[0] = z[3];
     f[1] = z[4];
     f[2] = z[5];
-    f[3] = 0.0;
-    f[4] = 0.0;
-    f[5] = 0.0;
-    if (act) {
-      const double x = z[0], yy = z[1], zz = z[2];
-      const double r2 = dot3(x, yy, zz, x, yy, zz);
-      const double r = sqrt(r2);
-      const double w0 = mu / (r2 * r);
-      double ax = -(w0 * x), ay = -(w0 * yy), az = -(w0 * zz);
-      const double loj = plo[a], wvj = pwv[a];
-      for (int b = 0; b < gs; ++b) {
-        if (b == a) continue;
-        const double dx = pos[3 * b + 0] - x, dy = pos[3 * b + 1] - yy, dz = pos[3 * b + 2] - zz;
-        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-        const double lok = plo[b], wvk = pwv[b];
-        const double w = close_weight(d2, eps2, pm[b], loj > lok ? loj : lok, wvj < wvk ? wvj : wvk);
-        ax = fma(w, dx, ax);
-        ay = fma(w, dy, ay);
-        az = fma(w, dz, az);
-      }
-      f[3] = ax;
-      f[4] = ay;
-      f[5] = az;
+    const double r2 = dot3(x, yy, zz, x, yy, zz);
+    const double r = sqrt(r2);
+    const double w0 = mu / (r2 * r);
+    double ax = -(w0 * x), ay = -(w0 * yy), az = -(w0 * zz);
+    for (int b = 0; b < gs; ++b) {
+      const double xb = __shfl(x, b, 64), yb = __shfl(yy, b, 64), zb = __shfl(zz, b, 64);
+      const double mb = __shfl(mj, b, 64), lob = __shfl(loj, b, 64), wvb = __shfl(wvj, b, 64);
+      if (b == a) continue;
+      const double dx = xb - x, dy = yb - yy, dz = zb - zz;
+      const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double w = close_weight(d2, eps2, mb, loj > lob ? loj : lob, wvj < wvb ? wvj : wvb);
+      ax = fma(w, dx, ax);
+      ay = fma(w, dy, ay);
+      az = fma(w, dz, az);
     }
-    __syncthreads();
+    f[3] = act ? ax : 0.0;
+    f[4] = act ? ay : 0.0;
+    f[5] = act ? az : 0.0;
   };
   uint32_t st = 0;
   double t = 0.0, H = h;
@@ -566,16 +596,18 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
   return st;
 }
 
-// Drift: workgroups [0, nK) drift the bodies in no group on their Kepler
-// orbits (and clear the flags for the next step); the rest walk the group list.
+// Drift: workgroups [0, nK) drift the bodies in no pair on their Kepler
+// orbits (and clear the flags for the next step); each of the others builds
+// the encounter groups from the step's pair list in its LDS and takes every
+// nG-th group (LDS as build_groups lists).
 __global__ void __launch_bounds__(64)
     mb_drift_kernel(MBShape sh, int nK, int nG, const double *__restrict__ bp_all,
                     const double *__restrict__ y_all, const double *__restrict__ va_all,
                     double *__restrict__ xb_all, double *__restrict__ vb_all,
-                    int *__restrict__ flag_all, const unsigned int *__restrict__ ctl_all,
-                    const int *__restrict__ gstart_all, const int *__restrict__ gmem_all,
-                    unsigned int *__restrict__ status, unsigned long long *__restrict__ stats,
-                    StepConst sc) {
+                    int *__restrict__ flag_all, const int *__restrict__ edge_all,
+                    const unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
+                    unsigned long long *__restrict__ stats, StepConst sc) {
+  extern __shared__ int ilds[];
   const int N = sh.N, s = blockIdx.y;
   const double *Y = y_all + (size_t)s * N * 3;
   const double *VA = va_all + (size_t)s * N * 3;
@@ -602,52 +634,65 @@ __global__ void __launch_bounds__(64)
     VB[3 * i + 2] = vz;
     return;
   }
-  __shared__ double pos[3 * 64], pm[64], plo[64], pwv[64];
+  const int ne = (int)ctl_all[4 * s];
+  const bool lead = (int)blockIdx.x == nK && threadIdx.x == 0;
+  if (ne == 0) return;
+  if (ne > MB_EMAX) {
+    if (lead) atomicOr(status + s, ST_LIMIT);
+    return;
+  }
+  unsigned int *bm = (unsigned int *)ilds;
+  int *pre = ilds + 64, *fl = pre + 64, *L = fl + 2 * MB_EMAX, *ce = L + 2 * MB_EMAX;
+  int *cnt = ce + 2 * MB_EMAX, *gst = cnt + 2 * MB_EMAX, *gm = gst + 2 * MB_EMAX + 1;
+  bool big = false;
+  const int G = build_groups(N, ne, edge_all + (size_t)s * MB_EMAX * 2, bm, pre, fl, L, ce, cnt, gst, gm, big);
+  if (lead) {
+    atomicOr(status + s, big ? (ST_MASSIVE_ENC | ST_LIMIT) : ST_MASSIVE_ENC);
+    if (stats) {
+      atomicAdd(stats + S_MGROUPS, (unsigned long long)G);
+      atomicAdd(stats + S_MGBODIES, (unsigned long long)gst[G]);
+    }
+  }
   const BP bp = bp_of(bp_all, s, N);
-  const unsigned int *ctl = ctl_all + 4 * s;
-  const int *gstart = gstart_all + (size_t)s * (N + 1);
-  const int *gmem = gmem_all + (size_t)s * N;
-  const int G = (int)ctl[2];
   const int a = threadIdx.x;
   unsigned int nrhs = 0;
   uint32_t st = 0;
   for (int g = (int)blockIdx.x - nK; g < G; g += nG) {
-    const int g0 = gstart[g], gs = gstart[g + 1] - g0;
+    const int g0 = gst[g], gs = gst[g + 1] - g0;
     if (gs > MB_GMAX) continue;
-    const int j = a < gs ? gmem[g0 + a] : -1;
-    double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    pm[a] = 0.0;
-    plo[a] = 0.0;
-    pwv[a] = 0.0;
+    const int j = a < gs ? gm[g0 + a] : -1;
+    // lanes past the group carry a unit position so their own (unused)
+    // Sun term stays finite
+    double y[6] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double mj = 0.0, loj = 0.0, wvj = 0.0;
     if (j >= 0) {
       for (int c = 0; c < 3; ++c) {
         y[c] = Y[3 * j + c];
         y[3 + c] = VA[3 * j + c];
       }
-      pm[a] = bp.m[j];
-      plo[a] = bp.lo[j];
-      pwv[a] = bp.winv[j];
+      mj = bp.m[j];
+      loj = bp.lo[j];
+      wvj = bp.winv[j];
     }
-    __syncthreads();
-    st |= bs_group_wave(y, a, gs, sc.h, sc.mu, sc.eps2, sc.tol, pm, plo, pwv, pos, nrhs);
+    st |= bs_group_wave(y, a, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
     if (j >= 0)
       for (int c = 0; c < 3; ++c) {
         XB[3 * j + c] = y[c];
         VB[3 * j + c] = y[3 + c];
       }
-    __syncthreads();
   }
   if (st && a == 0) atomicOr(status + s, st);
 }
 
 size_t close_lds(const MBShape &sh) {
-  return ((size_t)sh.nch * sh.TI * 3 + (size_t)sh.nch * 3 + 3) * sizeof(double);
+  return ((size_t)sh.nch * sh.TI * 3 + (size_t)sh.nch * 3 + 3 + (size_t)6 * sh.CS * MB_CHUNK) * sizeof(double);
 }
 
 size_t open_lds(const MBShape &sh) {
-  return ((size_t)sh.nch * 3 + 3) * sizeof(double) +
-         ((size_t)3 * sh.N + 2 * MB_EMAX + 64) * sizeof(int);
+  return ((size_t)sh.nch * 3 + 3 + (size_t)5 * sh.CS * MB_CHUNK) * sizeof(double);
 }
+
+size_t drift_lds(const MBShape &) { return ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int); }
 
 int launch_close(Ctx *c, bool init, int kicks, int last) {
   const MBShape sh = mb_shape(c->NM);
@@ -655,11 +700,11 @@ int launch_close(Ctx *c, bool init, int kicks, int last) {
   const StepConst sc = c->step_const();
   if (init)
     mb_close_kernel<true><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
-        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_status, sc,
+        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_mctl, c->d_status, sc,
         c->cfg.central_mass);
   else
     mb_close_kernel<false><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
-        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_status, sc,
+        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_mctl, c->d_status, sc,
         c->cfg.central_mass);
   return c->hip(hipGetLastError(), "many-body kick launch");
 }
@@ -669,8 +714,8 @@ int launch_open(Ctx *c) {
   const dim3 grid((unsigned int)((sh.N + sh.TI - 1) / sh.TI), (unsigned int)c->S);
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
   mb_open_kernel<<<grid, MB_BLOCK, open_lds(sh), c->stream>>>(
-      sh, c->d_bp, c->d_mx, c->d_mv, c->d_my, c->d_mflag, c->d_medge, c->d_mctl, c->d_gstart,
-      c->d_gmem, c->d_status, stats, c->step_const(), c->cfg.central_mass);
+      sh, c->d_bp, c->d_mx, c->d_mv, c->d_my, c->d_mflag, c->d_medge, c->d_mctl, c->d_status, stats,
+      c->step_const(), c->cfg.central_mass);
   return c->hip(hipGetLastError(), "many-body encounter launch");
 }
 
@@ -679,9 +724,10 @@ int launch_drift(Ctx *c) {
   const int nK = (sh.N + 63) / 64;
   const dim3 grid((unsigned int)(nK + MB_DRIFT_GROUP_BLOCKS), (unsigned int)c->S);
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  mb_drift_kernel<<<grid, 64, 0, c->stream>>>(sh, nK, MB_DRIFT_GROUP_BLOCKS, c->d_bp, c->d_my, c->d_mv,
-                                              c->d_mxb, c->d_mvb, c->d_mflag, c->d_mctl, c->d_gstart,
-                                              c->d_gmem, c->d_status, stats, c->step_const());
+  mb_drift_kernel<<<grid, 64, drift_lds(sh), c->stream>>>(sh, nK, MB_DRIFT_GROUP_BLOCKS, c->d_bp, c->d_my,
+                                                          c->d_mv, c->d_mxb, c->d_mvb, c->d_mflag,
+                                                          c->d_medge, c->d_mctl, c->d_status, stats,
+                                                          c->step_const());
   return c->hip(hipGetLastError(), "many-body drift launch");
 }
 
@@ -755,8 +801,6 @@ int mb_alloc(Ctx *c) {
   if ((rc = c->dalloc((void **)&c->d_mflag, (size_t)S * N * sizeof(int), "encounter flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_medge, (size_t)S * MB_EMAX * 2 * sizeof(int), "encounter pairs"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mctl, (size_t)S * 4 * sizeof(unsigned int), "group counters"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_gstart, (size_t)S * (N + 1) * sizeof(int), "group starts"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_gmem, (size_t)S * N * sizeof(int), "group members"))) return rc;
   return MINT_OK;
 }
 
@@ -794,9 +838,9 @@ int run_many(Ctx *c, int64_t nsteps) {
     return rc;
   for (int s = 0; s < c->S; ++s)
     if (st[s] & ST_LIMIT)
-      return c->fail(MINT_ELIMIT, "many-body engine: system " + std::to_string(s) +
-                                      " had more than 2048 encounter pairs or an encounter group of more "
-                                      "than 64 bodies in one step; its state is not valid");
+      return c->fail(MINT_ELIMIT, "many-body engine: system " + std::to_string(s) + " had more than " +
+                                      std::to_string(MB_EMAX) + " encounter pairs or an encounter group of more "
+                                      "than " + std::to_string(MB_GMAX) + " bodies in one step; its state is not valid");
   return MINT_OK;
 }
 

@@ -8,17 +8,21 @@ import sqlite3
 import sys
 
 
+def short(name):
+    """Kernel name without the argument list (and without anonymous namespaces)."""
+    return name.replace("(anonymous namespace)::", "").split("(")[0]
+
+
 def trace(db):
     cur = sqlite3.connect(db).cursor()
     print("| kernel | calls | total ms | avg ms | % |")
     print("|---|---|---|---|---|")
     for name, n, tot, avg, pct in cur.execute(
             "select name,total_calls,total_duration,average,percentage from top_kernels"):
-        short = name.split("(")[0]
-        print(f"| `{short}` | {n} | {tot / 1e3:.3f} | {avg / 1e3:.4f} | {pct:.2f} |")
-    for (short, vg, ag, sg, scr) in cur.execute(
+        print(f"| `{short(name)}` | {n} | {tot / 1e3:.3f} | {avg / 1e3:.4f} | {pct:.2f} |")
+    for (name, vg, ag, sg, scr) in cur.execute(
             "select distinct name, vgpr_count, accum_vgpr_count, sgpr_count, scratch_size from kernels"):
-        print(f"resources `{short.split('(')[0]}`: vgpr {vg} agpr {ag} sgpr {sg} scratch {scr}")
+        print(f"resources `{short(name)}`: vgpr {vg} agpr {ag} sgpr {sg} scratch {scr}")
 
 
 def counter(db, filt=""):
@@ -29,7 +33,7 @@ def counter(db, filt=""):
     print("|---|---|---|---|")
     for k, c, v, n, d in rows:
         if filt in k:
-            print(f"| `{k.split('(')[0]}` | {c} | {v:.6g} | {n} |")
+            print(f"| `{short(k)}` | {c} | {v:.6g} | {n} |")
 
 
 if __name__ == "__main__":

@@ -132,3 +132,25 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
         out = g.download()
     assert np.array_equal(out.tx, ref.tx) and np.array_equal(out.tv, ref.tv)
     assert np.array_equal(out.mx, ref.mx) and np.array_equal(out.mv, ref.mv)
+
+
+def test_embryos_1000_full_run_1e4_steps():
+    """The all-massive embryo configuration at full length: 1000 bodies, 1e4
+    steps of 0.002 yr on the many-body engine.  Size-independent checks: the
+    state stays finite, no capacity limit or BS budget is hit, encounter
+    groups formed, and the energy error stays bounded."""
+    st = W.embryos(1000)
+    with mint.Mint(dt=0.002, bs_tol=1e-11, collect_stats=1) as g:
+        g.upload(st)
+        g.set_radii(st.meta["radius"])
+        E0 = g.energy()
+        g.step(10000)
+        E1 = g.energy()
+        status = g.status()
+        s = g.stats()
+        out = g.download()
+    print("embryos 1e4 steps:", s, "dE/E", (E1 - E0) / E0, flush=True)
+    assert np.all(np.isfinite(out.mx)) and np.all(np.isfinite(out.mv))
+    assert not status[0] & (mint.ST_NAN | mint.ST_NEWTON | mint.ST_BS_DEPTH | mint.ST_LIMIT)
+    assert s["massive_groups"] > 1000
+    assert abs((E1[0] - E0[0]) / E0[0]) < 1e-9
