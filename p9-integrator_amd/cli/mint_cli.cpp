@@ -14,7 +14,12 @@
 //      src/simutils.cuh:299).
 //   optional keys: "dt" (default 0.8219, src/mercurius.cu:43), "n_hill",
 //      "bs_tol", "softening2", "central_mass", "form" (plain|flat|batched),
-//      "checkpoint_every" (steps between output checkpoints; default: end only).
+//      "checkpoint_every" (steps between output checkpoints; default: end only),
+//      per-body "radius" [AU] and top-level "density" [g/cm^3] (physical
+//      radii, recorded and checked for overlaps, never merged; a body without
+//      "radius" gets (3 m / (4 pi density))^(1/3) when "density" is given).
+//   systems of more than 8 massive bodies (up to 2048) run when they carry no
+//      test particles (the all-massive embryo configuration).
 // -s / -r (additions): write a bit-exact binary checkpoint at the end of the
 // run / resume from one (the state replaces the configuration's initial state;
 // -c still names the bodies for the output files).
@@ -39,6 +44,7 @@ namespace {
 struct Body {
   std::string name;
   double mass, a, e, inc, node, argp, manom;
+  double radius;  // -1: not given
 };
 
 Body body_from(const jsonmin::Value &v, const std::string &dflt_name) {
@@ -51,6 +57,7 @@ Body body_from(const jsonmin::Value &v, const std::string &dflt_name) {
   b.node = v.at("longitude_of_ascending_node").number();
   b.argp = v.at("argument_of_perihelion").number();
   b.manom = v.number_or("mean_anomaly", 0.0);
+  b.radius = v.number_or("radius", -1.0);
   return b;
 }
 
@@ -163,8 +170,9 @@ int main(int argc, char **argv) {
     NM += m > 1e-12;
   }
   const int NT = NB - NM;
-  if (NM < 1 || NM > 8) {
-    std::cerr << "mint: " << NM << " massive bodies; this build integrates 1 to 8 massive bodies per system\n";
+  if (NM < 1 || NM > 2048 || (NM > 8 && NT > 0)) {
+    std::cerr << "mint: " << NM << " massive and " << NT << " massless bodies; this build integrates 1 to 8 "
+                 "massive bodies per system with test particles, up to 2048 without\n";
     return 1;
   }
   for (const Body &s : sweeps)
@@ -211,6 +219,25 @@ int main(int argc, char **argv) {
   mint_state st{S, NM, off.data(), m.data(), mx.data(), mv.data(), t[0].data(), t[1].data(), t[2].data(),
                 t[3].data(), t[4].data(), t[5].data(), nullptr};
   check(mint_upload(ctx, &st), ctx, "upload");
+  // physical radii: per-body "radius" [AU], else from "density" [g/cm^3]
+  const double density = cfgj.number_or("density", -1.0);
+  bool any_radius = density > 0.0;
+  for (const Body &b : bodies) any_radius = any_radius || b.radius >= 0.0;
+  for (const Body &b : sweeps) any_radius = any_radius || b.radius >= 0.0;
+  if (any_radius) {
+    const double au_cm = 1.495978707e13, msun_g = 1.98847e33;
+    const double rho = density > 0.0 ? density * au_cm * au_cm * au_cm / msun_g : -1.0;  // Msun / AU^3
+    std::vector<double> rad((size_t)S * NM, 0.0);
+    for (int s = 0; s < S; ++s)
+      for (int k = 0; k < NM; ++k) {
+        const int i = order[k];
+        const Body &b = (i == 0 && !sweeps.empty()) ? sweeps[s] : bodies[i];
+        rad[(size_t)s * NM + k] = b.radius >= 0.0 ? b.radius
+                                  : rho > 0.0    ? std::cbrt(3.0 * b.mass / (4.0 * M_PI * rho))
+                                                 : 0.0;
+      }
+    check(mint_set_radii(ctx, rad.data()), ctx, "radii");
+  }
   int64_t step0 = 0;
   if (!args.resume.empty()) {
     check(mint_load_checkpoint(ctx, args.resume.c_str(), &step0), ctx, "resume");
@@ -271,10 +298,10 @@ int main(int argc, char **argv) {
       pf << "\n";
     }
     pf << "\n";
-    std::cout << "NaN counter after step " << done << ": " << nan << std::endl;
+    std::cout << "NaN counter after step " << step0 + done << ": " << nan << std::endl;
     // elements (src/mercurius.cu:210-248)
     std::ofstream ef("elements_" + std::to_string(args.device) + ".txt");
-    ef << "Timestep: " << done << "\n";
+    ef << "Timestep: " << step0 + done << "\n";
     for (int s = 0; s < S; ++s) {
       ef << "Experiment: " << s << "\n";
       for (int i = 0; i < NB; ++i) {

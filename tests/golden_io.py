@@ -25,6 +25,8 @@ def load(path):
     o = d["output"]
     out = dict(mx=_arr(o["mx"], (S, NM, 3)), mv=_arr(o["mv"], (S, NM, 3)), tx=_arr(o["tx"], (-1, 3)),
                tv=_arr(o["tv"], (-1, 3)), rc=_arr(o["rc"], (S, NM)), status=np.array(o["status"], np.uint32))
+    d.setdefault("bs_tol", 1e-12)
+    d["radii"] = None if d.get("radii") is None else _arr(d["radii"], (S, NM))
     return d, st, out
 
 

@@ -113,6 +113,31 @@ def test_cli_runs_a_reference_schema_config(tmp_path):
     assert el.startswith("Timestep: 40") and el.count("Experiment:") == 3
 
 
+def test_cli_embryos_with_density_and_resume(tmp_path):
+    """An all-massive configuration through the CLI (40 embryos, radii from a
+    density): a 40-step run equals a 20-step run saved with -s and resumed
+    with -r for 20 more, to the printed digit, and the step count continues."""
+    cli = os.path.join(ROOT, "p9-integrator_amd", "mint")
+    rng = np.random.default_rng(7)
+    bodies = [dict(name=f"E{i}", mass=3e-7 * rng.uniform(0.8, 1.2), semi_major_axis=rng.uniform(1.0, 1.05),
+                   eccentricity=rng.rayleigh(0.01), inclination=rng.rayleigh(0.005),
+                   longitude_of_ascending_node=rng.uniform(0, 6.28), argument_of_perihelion=rng.uniform(0, 6.28),
+                   mean_anomaly=rng.uniform(0, 6.28)) for i in range(40)]
+    (tmp_path / "e.json").write_text(json.dumps(dict(bodies=bodies, dt=0.002, bs_tol=1e-11, density=3.0)))
+
+    def run(*a):
+        r = subprocess.run([cli, "-c", "e.json", *a], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+
+    run("-t", "40", "-o", "full")
+    run("-t", "20", "-o", "part1", "-s", "ck.bin")
+    out = run("-t", "20", "-o", "part2", "-r", "ck.bin", "-i")
+    assert "Resumed from ck.bin at step 20" in out and "NaN counter after step 40: 0" in out
+    assert (tmp_path / "full").read_text() == (tmp_path / "part2").read_text()
+    assert (tmp_path / "elements_0.txt").read_text().startswith("Timestep: 40")
+
+
 def test_checkpoint_resume_is_bitwise(tmp_path):
     st = W.p9_sweep(systems=np.arange(5), tp_scale=1 / 4)
     path = str(tmp_path / "run.ckpt")

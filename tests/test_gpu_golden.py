@@ -16,8 +16,10 @@ def test_gpu_reproduces_golden(path, form):
     d, st, ref = load(path)
     print(f"golden {os.path.basename(path)} form={form}", flush=True)
     with mint.Mint(dt=d["dt"], form=form, kepler_guess=d["guess"], chunk_steps=32, block_size=128,
-                   tps_per_lane=2) as g:
+                   tps_per_lane=2, bs_tol=d["bs_tol"]) as g:
         g.upload(st)
+        if d["radii"] is not None:
+            g.set_radii(d["radii"])
         g.step(d["steps"])
         out = g.download()
         status = g.status()

@@ -111,7 +111,10 @@ def test_cli_usage_and_bad_json(tmp_path):
     assert r.returncode == 1 and "semi_major_axis" in r.stderr
     body = dict(name="b", mass=1e-6, semi_major_axis=1.0, eccentricity=0.0, inclination=0.0,
                 longitude_of_ascending_node=0.0, argument_of_perihelion=0.0, mean_anomaly=0.0)
-    r = _cli(["-c", "@CFG"], tmp_path, json.dumps({"bodies": [body] * 9}))
+    tp = dict(body, name="tp", mass=0.0, semi_major_axis=2.0)
+    r = _cli(["-c", "@CFG"], tmp_path, json.dumps({"bodies": [body] * 9 + [tp]}))
+    assert r.returncode == 1 and "massive" in r.stderr  # test particles need at most 8 massive bodies
+    r = _cli(["-c", "@CFG"], tmp_path, json.dumps({"bodies": [body] * 2049}))
     assert r.returncode == 1 and "massive" in r.stderr
     r = _cli(["-c", "@CFG", "-t"], tmp_path, json.dumps({"bodies": [body]}))
     assert r.returncode == 2
