@@ -1,7 +1,7 @@
 // mint_cli.cpp — the `mint` command line tool on top of libmint.
 //
 //   mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]
-//        [-s checkpoint_out] [-r checkpoint_in]
+//        [-s checkpoint_out] [-r checkpoint_in] [-n]
 //
 // Same flags and JSON schema as the reference driver (README.md:10-19;
 // args_parse src/simutils.cuh:138-175; sim_from_config_file
@@ -22,7 +22,8 @@
 //      test particles (the all-massive embryo configuration).
 // -s / -r (additions): write a bit-exact binary checkpoint at the end of the
 // run / resume from one (the state replaces the configuration's initial state;
-// -c still names the bodies for the output files).
+// -c still names the bodies for the output files).  -n (addition): read the
+// configuration, print what would run, and exit without touching a GPU.
 // Output files keep the reference formats: positions appended to output_<dev>
 // (src/simutils.cuh:186-230) and elements in elements_<dev>.txt
 // (src/mercurius.cu:210-248; src/simutils.cuh:177-184).
@@ -65,12 +66,12 @@ struct Args {
   std::string config, output, save, resume;
   long long steps = 1;
   int device = 0;
-  bool info = false, print_positions = false;
+  bool info = false, print_positions = false, dry = false;
 };
 
 int usage() {
   std::cerr << "usage: mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p] "
-               "[-s checkpoint_out] [-r checkpoint_in]\n";
+               "[-s checkpoint_out] [-r checkpoint_in] [-n]\n";
   return 2;
 }
 
@@ -88,6 +89,8 @@ bool parse_args(int argc, char **argv, Args &a) {
       a.info = true;
     } else if (f == "-p") {
       a.print_positions = true;
+    } else if (f == "-n") {
+      a.dry = true;
     } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d" || f == "-s" || f == "-r") {
       const char *v = need(f == "-t" || f == "-d" ? "a number" : "a file");
       if (!v) return false;
@@ -211,6 +214,20 @@ int main(int argc, char **argv) {
   }
   off[S] = (int64_t)S * NT;
 
+  if (args.dry) {
+    double msum = 0.0;
+    for (double v : m) msum += v;
+    std::cout << "Config " << args.config << ": " << S << " system(s), " << NM << " massive and " << NT
+              << " massless bodies each, dt " << cfg.dt << " yr, n_hill " << cfg.n_hill << ", bs_tol "
+              << cfg.bs_tol << ", form " << form << ", total massive mass " << msum << "\n";
+    if (args.info)
+      for (int k = 0; k < NB; ++k) {
+        const Body &b = bodies[order[k]];
+        std::cout << "Body " << order[k] << " " << b.name << ": mass " << b.mass << " a " << b.a << " e " << b.e
+                  << " i " << b.inc << "\n";
+      }
+    return 0;
+  }
   void *ctx = nullptr;
   if (mint_create(&cfg, &ctx) != 0) {
     std::cerr << "mint: " << mint_last_error(nullptr) << "\n";
