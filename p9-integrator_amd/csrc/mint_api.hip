@@ -298,6 +298,7 @@ void mint_destroy(void *ctx) {
   for (hipEvent_t e : c->ev_side) hipEventDestroy(e);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (hipEvent_t e : c->ev_eph) hipEventDestroy(e);
   for (hipEvent_t e : c->ev_tp) hipEventDestroy(e);
   hipStreamDestroy(c->stream);

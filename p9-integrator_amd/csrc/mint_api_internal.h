@@ -75,6 +75,7 @@ struct Ctx {
   int *d_mflag = nullptr;           // [S][NM] body is in an encounter pair this step
   int *d_medge = nullptr;           // [S][MB_EMAX][2] encounter pairs
   unsigned int *d_mctl = nullptr;   // [S][4]: [0] pairs in the list this step
+  std::vector<hipEvent_t> tev;       // event pool of the per-kernel timing mode
   hipGraphExec_t mb_graph = nullptr;
   long long mb_graph_steps = 0;
 
@@ -92,32 +93,42 @@ struct Ctx {
 inline bool timing_on(const Ctx *c) { return (c->cfg.collect_stats & 2) != 0; }
 
 // HIP-event timer around single launches on the context stream (profiling
-// runs only: stop() synchronises).
+// runs only).  Event pairs are recorded without waiting and resolved when the
+// timer is flushed (at the latest on destruction), so the launches still run
+// back to back and each pair brackets one kernel.
 struct KTimer {
   Ctx *c;
-  hipEvent_t a = nullptr, b = nullptr;
   bool on;
-  explicit KTimer(Ctx *cc) : c(cc), on(timing_on(cc)) {
-    if (on) {
-      hipEventCreate(&a);
-      hipEventCreate(&b);
+  std::vector<std::pair<double *, int64_t *>> dst;
+  explicit KTimer(Ctx *cc) : c(cc), on(timing_on(cc)) {}
+  ~KTimer() { flush(); }
+  hipEvent_t ev(size_t i) {
+    while (c->tev.size() <= i) {
+      hipEvent_t e = nullptr;
+      hipEventCreate(&e);
+      c->tev.push_back(e);
     }
-  }
-  ~KTimer() {
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    return c->tev[i];
   }
   void start() {
-    if (on) hipEventRecord(a, c->stream);
+    if (on) hipEventRecord(ev(2 * dst.size()), c->stream);
   }
   void stop(double &ms, int64_t &n) {
     if (!on) return;
-    hipEventRecord(b, c->stream);
-    hipEventSynchronize(b);
-    float t = 0.f;
-    hipEventElapsedTime(&t, a, b);
-    ms += t;
-    n += 1;
+    hipEventRecord(ev(2 * dst.size() + 1), c->stream);
+    dst.emplace_back(&ms, &n);
+    if (dst.size() >= 1024) flush();  // bound the event pool
+  }
+  void flush() {
+    if (!on || dst.empty()) return;
+    hipEventSynchronize(c->tev[2 * dst.size() - 1]);
+    for (size_t i = 0; i < dst.size(); ++i) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, c->tev[2 * i], c->tev[2 * i + 1]);
+      *dst[i].first += t;
+      *dst[i].second += 1;
+    }
+    dst.clear();
   }
 };
 
