@@ -368,7 +368,7 @@ __device__ __forceinline__ bool encounter_tp(double x, double y, double z,
     const double dy = Y[3 * j + 1] - y;
     const double dz = Y[3 * j + 2] - z;
     const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-    f = f || (r2 < prm[P_RC2(NM) + j]);
+    f |= (r2 < prm[P_RC2(NM) + j]);  // every pair, no short-circuit branches
   }
   return f;
 }

@@ -427,7 +427,14 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
     if (act[p]) accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2]);
 
   for (int k = 0; k < C; ++k) {
-    const double *r = rsys + (size_t)k * REC;
+    // The step's whole ephemeris record is loaded up front (uniform, so one
+    // round of scalar loads whose latency overlaps the kick); loaded where
+    // each value is used, the loads would sit behind the divergent branches
+    // and each wait would be exposed.
+    const double *rg = rsys + (size_t)__builtin_amdgcn_readfirstlane(k) * REC;
+    double r[REC];
+#pragma unroll
+    for (int i = 0; i < REC; ++i) r[i] = rg[i];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       if (!act[p]) continue;
