@@ -267,18 +267,16 @@ def main():
     shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
     if shape:
         g = make_ctx(form, guess, scorer, block_size=shape[0], tps_per_lane=shape[1], chunk_steps=shape[2])
+    elif scorer:
+        # the scorer is trained for one launch window: tune block and TPs per lane at that window
+        import train_learned as T
+
+        g = make_ctx(form, guess, scorer, chunk_steps=T.window())
     else:
         g = make_ctx(form, guess, scorer)
     if form == mint.FORM_BATCHED and not shape:
         g.upload(st)
         shape = g.autotune(args.tune_cache)
-        if scorer:
-            # the scorer is trained for one launch window; keep the tuned block / TPs per lane
-            import train_learned as T
-
-            shape = (shape[0], shape[1], T.window())
-            g.close()
-            g = make_ctx(form, guess, scorer, block_size=shape[0], tps_per_lane=shape[1], chunk_steps=shape[2])
         log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]}")
     wall, ev_s, E0, E1 = timed_run(g, st, args.steps, args.warmup)
     status = g.status()

@@ -216,9 +216,12 @@ int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const doub
                                double *inc, double *node, double *argp, double *mean_anom,
                                double *ecc, double *sma);
 
-/* Autotuner (batched form): time the candidate launch shapes on a sample of the
- * uploaded batch, keep the fastest, and store it in cache_path (NULL = no file).
- * The state on the device is left unchanged.  best[3] = {block, tps/lane, chunk}. */
+/* Autotuner (batched form): time the candidate launch shapes on the uploaded
+ * batch (each once untimed, then a timed 256-step probe), keep the fastest,
+ * and store it in cache_path (NULL = no file).  A shape parameter set in the
+ * configuration (block_size, tps_per_lane, chunk_steps > 0) is held at that
+ * value.  The state on the device is left unchanged.
+ * best[3] = {block, tps/lane, chunk}. */
 int mint_autotune(void *ctx, const char *cache_path, int32_t *best);
 
 /* Learned parts (FP32).  Weights live in the context.

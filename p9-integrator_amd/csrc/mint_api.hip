@@ -53,8 +53,11 @@ void Ctx::free_device() {
   }
   graphs.clear();
   d_prm = d_mx = d_mv = d_xs0 = d_rec = nullptr;
-  d_xs0b = d_recb = nullptr;
+  d_xs0b = d_recb = d_xs0c = d_recc = nullptr;
   d_pc = nullptr;
+  d_carry = nullptr;
+  d_slist = nullptr;
+  d_q2 = nullptr;
   d_need = nullptr;
   rec2_steps_alloc = 0;
   for (auto &p : d_tp) p = nullptr;
@@ -110,10 +113,14 @@ int Ctx::ensure_rec(int C) {
   return MINT_OK;
 }
 
-// second ephemeris buffer (and chunk-start positions) for the pipeline
+// the batched form's other buffers: ephemeris buffers 2 and 3 (with their
+// chunk-start positions) and the odd windows' queue
 int Ctx::ensure_rec2(int C) {
+  int rc;
+  if (!d_q2 && (rc = dalloc((void **)&d_q2, (size_t)std::max<long long>(Ntp, 1) * sizeof(QEntry), "BS queue (odd windows)")))
+    return rc;
   if (C <= rec2_steps_alloc) return MINT_OK;
-  for (double **p : {&d_recb, &d_xs0b}) {
+  for (double **p : {&d_recb, &d_xs0b, &d_recc, &d_xs0c}) {
     if (*p) {
       hipFree(*p);
       allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)*p));
@@ -124,10 +131,11 @@ int Ctx::ensure_rec2(int C) {
     if (g.exec) hipGraphExecDestroy(g.exec);
   graphs.clear();
   const size_t bytes = (size_t)S * (size_t)C * (size_t)rec_size(NM) * sizeof(double);
-  int rc = dalloc((void **)&d_recb, bytes, "ephemeris (second buffer)");
-  if (rc) return rc;
-  if ((rc = dalloc((void **)&d_xs0b, (size_t)S * NM * 3 * sizeof(double), "chunk start positions (second buffer)")))
-    return rc;
+  const size_t xb = (size_t)S * NM * 3 * sizeof(double);
+  if ((rc = dalloc((void **)&d_recb, bytes, "ephemeris (second buffer)"))) return rc;
+  if ((rc = dalloc((void **)&d_recc, bytes, "ephemeris (third buffer)"))) return rc;
+  if ((rc = dalloc((void **)&d_xs0b, xb, "chunk start positions (second buffer)"))) return rc;
+  if ((rc = dalloc((void **)&d_xs0c, xb, "chunk start positions (third buffer)"))) return rc;
   rec2_steps_alloc = C;
   return MINT_OK;
 }
@@ -301,6 +309,8 @@ void mint_destroy(void *ctx) {
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (hipEvent_t e : c->ev_eph) hipEventDestroy(e);
   for (hipEvent_t e : c->ev_tp) hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_fork) hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_join) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
   hipStreamDestroy(c->stream2);
   hipStreamDestroy(c->stream3);
@@ -426,10 +436,12 @@ int mint_upload(void *ctx, const mint_state *st) {
   for (int d = 0; d < 6; ++d)
     if ((rc = c->dalloc((void **)&c->d_tp[d], (size_t)N * sizeof(double), "TP state"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_tp_sys, (size_t)N * sizeof(int), "TP system index"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_pre, (size_t)N, "pre-bucket flags"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_need, (size_t)S * sizeof(unsigned int), "cache flags"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_pre, (size_t)N, "queue-head flags"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_carry, (size_t)N, "carry flags"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_need, (size_t)2 * S * sizeof(unsigned int), "cache flags"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_slist, (size_t)4 * S * sizeof(int), "cache system lists"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_q, (size_t)std::max<int64_t>(N, 1) * sizeof(QEntry), "BS queue"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_qctl, 4 * sizeof(unsigned int), "queue counters"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_qctl, 8 * sizeof(unsigned int), "queue counters"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_kep, KM_NP * sizeof(float), "kepler mlp"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_sc, SC_NP * sizeof(float), "scorer"))) return rc;
   }
@@ -454,7 +466,8 @@ int mint_upload(void *ctx, const mint_state *st) {
     c->timing = mint_timing{};
     return MINT_OK;
   }
-  if ((rc = c->hip(hipMemset(c->d_qctl, 0, 4 * sizeof(unsigned int)), "queue clear"))) return rc;
+  if ((rc = c->hip(hipMemset(c->d_qctl, 0, 8 * sizeof(unsigned int)), "queue clear"))) return rc;
+  if (N > 0 && (rc = c->hip(hipMemset(c->d_carry, 0, (size_t)N), "carry clear"))) return rc;
   // networks: keep any weights set earlier on this context
   if (!c->w_kep.empty())
     hipMemcpy(c->d_nnw_kep, c->w_kep.data(), KM_NP * sizeof(float), hipMemcpyHostToDevice);

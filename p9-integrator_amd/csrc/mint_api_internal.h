@@ -23,10 +23,12 @@ struct Ctx {
   std::string err;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // ephemeris runs one window ahead on this stream
-  hipStream_t stream3 = nullptr;   // pre-bucketed encounter work beside the main kernel
+  hipStream_t stream3 = nullptr;   // queue work (window heads, carried tails) beside the main kernel
   std::vector<hipEvent_t> ev_side;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> ev_eph, ev_tp;  // per-window completion events of the pipeline
+  // per-window events of the pipeline: ephemeris done, window done, queue
+  // work forked from / joined back into the context stream
+  std::vector<hipEvent_t> ev_eph, ev_tp, ev_fork, ev_join;
 
   // batch shape
   int S = 0, NM = 0;
@@ -39,17 +41,23 @@ struct Ctx {
   // device buffers
   std::vector<void *> allocs;
   double *d_prm = nullptr, *d_mx = nullptr, *d_mv = nullptr, *d_xs0 = nullptr, *d_rec = nullptr;
-  double *d_xs0b = nullptr, *d_recb = nullptr;  // second buffer of the pipelined batched form
-  double *d_pc = nullptr;          // perturber cache of the current window
-  unsigned int *d_need = nullptr;  // per-system: a TP entered the encounter queue this window
+  // ephemeris buffers 2 and 3 of the batched form (three rotate: a window's
+  // buffer is in use until its carried tails finish in the next window)
+  double *d_xs0b = nullptr, *d_recb = nullptr, *d_xs0c = nullptr, *d_recc = nullptr;
+  double *d_pc = nullptr;          // perturber cache of the window being served
+  unsigned int *d_need = nullptr;  // [2][S] per window parity: 1 queue head, 2 mid-window deferral
+  int *d_slist = nullptr;          // [2][2][S] per parity: systems flagged 1, then flagged 2
   double *d_tp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int *d_tp_sys = nullptr;
-  unsigned char *d_pre = nullptr;  // per-TP pre-bucket flag of the current window
+  unsigned char *d_pre = nullptr;    // per-TP: starts the current window in the queue
+  unsigned char *d_carry = nullptr;  // per-TP: deferred this window, queued for the next
   Tile *d_tiles = nullptr;
   int ntiles = 0;
   long long tiles_key = -1;
-  QEntry *d_q = nullptr;
-  unsigned int *d_qctl = nullptr;  // [0] entries so far, [2] entries from the pre-bucketing pass
+  QEntry *d_q = nullptr, *d_q2 = nullptr;  // BS queues of even / odd windows
+  // [2][4] per window parity: [0] entries so far, [1] systems flagged 1,
+  // [2] entries of the queue head, [3] systems flagged 2
+  unsigned int *d_qctl = nullptr;
   unsigned int *d_status = nullptr;
   unsigned long long *d_stats = nullptr;
   float *d_nnw_kep = nullptr, *d_nnw_sc = nullptr;

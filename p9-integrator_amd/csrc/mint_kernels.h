@@ -10,6 +10,13 @@
 //                     its lane goes idle for the rest of the launch
 //  tp_encounter_kernel  consumes the BS queue: each lane takes one deferred TP
 //                     and carries it to the end of the launch window
+//  tp_prebucket_kernel  builds a window's queue head: TPs deferred in the
+//                     previous window (carried) and, with the scorer, TPs
+//                     likely to meet a massive body within the window
+//
+// A TP deferred mid-window is finished by the encounter kernel after the main
+// kernel; with a next window in the same call it also starts that window in
+// the queue ("carried"), so its tail can run beside the next main kernel.
 #pragma once
 #include "mint_device.h"
 #include "mint_nn.h"
@@ -22,8 +29,9 @@ struct Tile {
   long long start;
 };
 
-// One deferred TP (64 B).  phase 0: state at the start of step k0 (before the
-// opening kick); phase 1: state after jump 1 of step k0, encounter flagged.
+// One deferred TP (64 B).  phase 0: the TP starts the window in the queue; its
+// state is read from the TP arrays when the entry is served (s unused);
+// phase 1: state after jump 1 of step k0, encounter flagged.
 struct QEntry {
   int tp;
   int sys;
@@ -89,6 +97,24 @@ __device__ __forceinline__ void kepler_features(const KepStart &k, double mu, do
   f[3] = fminf(fmaxf(ra, -2.0f), 2.0f);
 }
 
+// Learned correction factor 1 + c tau^3 of the series guess.  Out of line:
+// the network's weights and activations stay out of the register budget of
+// the step loops that call it (the call is taken only above the gate).
+static __device__ __noinline__ double learned_guess_factor(double r0, double eta0, double beta, double zeta0,
+                                                    double mu, double dt, const float *__restrict__ nnw) {
+  KepStart k;
+  k.r0 = r0;
+  k.ir0 = 0.0;
+  k.eta0 = eta0;
+  k.beta = beta;
+  k.zeta0 = zeta0;
+  float f[4];
+  double tau;
+  kepler_features(k, mu, dt, f, tau);
+  const double c = (double)mlp_kepler_fwd(nnw, f);
+  return fma(c, tau * (tau * tau), 1.0);
+}
+
 // Starting guess of the TP Kepler solve.  GUESS: 0 crude, 1 series, 2 learned
 // (series guess times (1 + c tau^3), c from the MLP, where tau > KM_TAU_GATE).
 template <int GUESS>
@@ -99,13 +125,8 @@ __device__ __forceinline__ double tp_guess(const KepStart &k, double mu, double 
   if (GUESS == 2) {
     // gate tau = dt sqrt(mu / r0^3) > KM_TAU_GATE, tested without sqrt or division
     const double ir3 = k.ir0 * (k.ir0 * k.ir0);
-    if ((dt * dt) * (mu * ir3) > KM_TAU_GATE * KM_TAU_GATE) {
-      float f[4];
-      double tau;
-      kepler_features(k, mu, dt, f, tau);
-      const double c = (double)mlp_kepler_fwd(nnw, f);
-      s = s * fma(c, tau * (tau * tau), 1.0);
-    }
+    if ((dt * dt) * (mu * ir3) > KM_TAU_GATE * KM_TAU_GATE)
+      s = s * learned_guess_factor(k.r0, k.eta0, k.beta, k.zeta0, mu, dt, nnw);
   }
   return s;
 }
@@ -251,37 +272,46 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
 }
 
 // ---------------------------------------------------------------------------
-// Perturber cache: one lane per (system, step, substep point, body), each a
-// Kepler drift of the body's drift-start state (Y, W) from the ephemeris.
-// Only systems with need[s] == want are computed: 1 = a TP of theirs was
-// pre-bucketed at the window start, 2 = a TP entered the queue mid-window.
+// Perturber cache: for each system of a list, each step of the window, each
+// substep point and each massive body, a Kepler drift of the body's
+// drift-start state (Y, W) from the ephemeris.  The lists hold the systems
+// whose TPs are in the encounter queue (built with the queue, each system
+// once), so the grid is fixed and small and its work scales with the flagged
+// systems only: beside the main kernel a sparse grid over every system would
+// wait for dispatch slots block by block.
 template <int NM>
-__global__ void __launch_bounds__(256) pcache_kernel(int s_begin, int s_end, int C,
+__global__ void __launch_bounds__(256) pcache_kernel(const int *__restrict__ list,
+                                                     const unsigned int *__restrict__ count, int C,
                                                      const double *__restrict__ rec,
-                                                     const unsigned int *__restrict__ need,
-                                                     unsigned int want,
                                                      double *__restrict__ pc, StepConst sc) {
   constexpr int REC = rec_size(NM);
-  const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NM;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int j = (int)(t % NM);
-  long long u = t / NM;
-  const int m = (int)(u % PC_PTS) + 1;
-  u /= PC_PTS;
-  const int k = (int)(u % C);
-  const int s = s_begin + (int)(u / C);
-  if (need[s] != want) return;
-  const double *r = rec + ((size_t)s * C + k) * REC;
-  double px = r[R_Y(NM) + 3 * j], py = r[R_Y(NM) + 3 * j + 1], pz = r[R_Y(NM) + 3 * j + 2];
-  double qx = r[R_W(NM) + 3 * j], qy = r[R_W(NM) + 3 * j + 1], qz = r[R_W(NM) + 3 * j + 2];
-  const double tau = (double)m * (sc.h / (double)PC_PTS);
-  bool ok;
-  kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, tau, ok);
-  double *o = pc + (((size_t)s * C + k) * PC_PTS + (m - 1)) * 3 * NM + 3 * j;
-  o[0] = px;
-  o[1] = py;
-  o[2] = pz;
+  const long long n = (long long)(*count) * C * PC_PTS * NM;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int j = (int)(t % NM);
+    long long u = t / NM;
+    const int m = (int)(u % PC_PTS) + 1;
+    u /= PC_PTS;
+    const int k = (int)(u % C);
+    const int s = list[u / C];
+    const double *r = rec + ((size_t)s * C + k) * REC;
+    double px = r[R_Y(NM) + 3 * j], py = r[R_Y(NM) + 3 * j + 1], pz = r[R_Y(NM) + 3 * j + 2];
+    double qx = r[R_W(NM) + 3 * j], qy = r[R_W(NM) + 3 * j + 1], qz = r[R_W(NM) + 3 * j + 2];
+    const double tau = (double)m * (sc.h / (double)PC_PTS);
+    bool ok;
+    kepler_drift_series(px, py, pz, qx, qy, qz, sc.mu, tau, ok);
+    double *o = pc + (((size_t)s * C + k) * PC_PTS + (m - 1)) * 3 * NM + 3 * j;
+    o[0] = px;
+    o[1] = py;
+    o[2] = pz;
+  }
+}
+
+// Flag system s for the perturber cache (need[s]: 0 none, 1 queue head,
+// 2 mid-window deferral) and list it the first time it is flagged.
+__device__ __forceinline__ void flag_system(unsigned int *need, unsigned int v, int s, int *list,
+                                           unsigned int *count) {
+  if (need[s] == 0u && atomicCAS(&need[s], 0u, v) == 0u) list[atomicAdd(count, 1u)] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -354,28 +384,35 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
 // ---------------------------------------------------------------------------
 // Scorer features of one (TP, massive body) pair at the start of a launch
 // window of duration T: distance, approach rate and closest approach in units
-// of r_c, and Hill-zone size ratios.
+// of r_c, and Hill-zone size ratios.  Differences are taken in FP64
+// (positions are hundreds of AU); the rest is FP32 with the hardware square
+// root, reciprocal square root and reciprocal (about one ulp each): the scorer
+// only ranks pairs, and the exact distance check decides every step.
 __device__ __forceinline__ void scorer_features(double x, double y, double z, double vx, double vy,
                                                 double vz, double X, double Y, double Z, double VX,
                                                 double VY, double VZ, double rc, double T,
                                                 float *f) {
-  const double dx = X - x, dy = Y - y, dz = Z - z;
-  const double ux = VX - vx, uy = VY - vy, uz = VZ - vz;
-  const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-  const double u2 = dot3(ux, uy, uz, ux, uy, uz);
-  const double du = dot3(dx, dy, dz, ux, uy, uz);
-  double tmin = u2 > 0.0 ? -du / u2 : 0.0;
-  tmin = fmin(fmax(tmin, 0.0), T);
-  const double mx = fma(ux, tmin, dx), my = fma(uy, tmin, dy), mz = fma(uz, tmin, dz);
-  const double rmin = sqrt(dot3(mx, my, mz, mx, my, mz));
-  const double R = sqrt(dot3(X, Y, Z, X, Y, Z));
-  const double rt = sqrt(dot3(x, y, z, x, y, z));
-  f[0] = (float)fmin(r / rc, 64.0) * 0.125f;
-  f[1] = (float)fmax(fmin(du / (r * rc) * T, 16.0), -16.0) * 0.25f;
-  f[2] = (float)fmin(sqrt(u2) * T / rc, 16.0) * 0.25f;
-  f[3] = (float)fmin(rmin / rc, 64.0) * 0.125f;
-  f[4] = (float)(rc / R) * 4.0f;
-  f[5] = (float)fmin(rt / R, 8.0) * 0.25f;
+  const float dx = (float)(X - x), dy = (float)(Y - y), dz = (float)(Z - z);
+  const float ux = (float)(VX - vx), uy = (float)(VY - vy), uz = (float)(VZ - vz);
+  const float Tf = (float)T, rcf = (float)rc, irc = __builtin_amdgcn_rcpf(rcf);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+  const float du = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  float tmin = u2 > 0.0f ? -du * __builtin_amdgcn_rcpf(u2) : 0.0f;
+  tmin = fminf(fmaxf(tmin, 0.0f), Tf);
+  const float mx = fmaf(ux, tmin, dx), my = fmaf(uy, tmin, dy), mz = fmaf(uz, tmin, dz);
+  const float rmin = __builtin_amdgcn_sqrtf(fmaf(mx, mx, fmaf(my, my, mz * mz)));
+  const float Xf = (float)X, Yf = (float)Y, Zf = (float)Z;
+  const float xf = (float)x, yf = (float)y, zf = (float)z;
+  const float iR = __builtin_amdgcn_rsqf(fmaf(Xf, Xf, fmaf(Yf, Yf, Zf * Zf)));
+  const float rt = __builtin_amdgcn_sqrtf(fmaf(xf, xf, fmaf(yf, yf, zf * zf)));
+  f[0] = fminf(r * irc, 64.0f) * 0.125f;
+  f[1] = fmaxf(fminf(((du * __builtin_amdgcn_rcpf(r)) * irc) * Tf, 16.0f), -16.0f) * 0.25f;
+  f[2] = fminf((__builtin_amdgcn_sqrtf(u2) * Tf) * irc, 16.0f) * 0.25f;
+  f[3] = fminf(rmin * irc, 64.0f) * 0.125f;
+  f[4] = (rcf * iR) * 4.0f;
+  f[5] = fminf(rt * iR, 8.0f) * 0.25f;
 }
 
 template <int NM, int P, int GUESS>
@@ -384,8 +421,9 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
     const double *__restrict__ prm_all, const double *__restrict__ xs0,
     const double *__restrict__ rec, int C, StepConst sc, const float *__restrict__ nnw,
     const unsigned char *__restrict__ pre, QEntry *__restrict__ q,
-    unsigned int *__restrict__ qcount, unsigned int *__restrict__ need,
-    unsigned int *__restrict__ status, unsigned long long *__restrict__ stats) {
+    unsigned int *__restrict__ qcount, unsigned int *__restrict__ need, int *__restrict__ slist,
+    unsigned char *__restrict__ carry, unsigned int *__restrict__ status,
+    unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so the tiles of one system share an L2.
@@ -448,7 +486,8 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
       if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
         const unsigned int slot = atomicAdd(qcount, 1u);
         q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
-        if (need[s] == 0u) need[s] = 2u;
+        flag_system(need, 2u, s, slist, qcount + 3);
+        if (carry != nullptr) carry[idx[p]] = 1;  // queued for the next window too
         act[p] = false;
         ++n_defer;
         n_idle += (unsigned)(C - k);
@@ -487,42 +526,56 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Learned pre-bucketing: one lane per TP.  A TP whose largest pair logit from
-// the scorer exceeds the threshold starts the window in the encounter queue
-// (phase 0) and is skipped by the main kernel.  The exact distance check
-// still decides every step inside the queue, so results do not change.
+// Queue head of a launch window: one lane per TP.  A TP carried from the
+// previous window (deferred there mid-window) starts this window in the
+// encounter queue; so does, with scorer weights, a TP whose largest pair
+// logit exceeds the threshold (learned pre-bucketing).  Queued TPs are
+// skipped by the main kernel.  The exact distance check still decides every
+// step inside the queue, so results do not change.  Carry flags are cleared
+// here; a carried TP's state is not read (its previous window may still be
+// finishing), the queue entry reads it when served.
 template <int NM>
 __global__ void __launch_bounds__(256) tp_prebucket_kernel(
     long long n, const int *__restrict__ tp_sys, TPArrays tp, const double *__restrict__ prm_all,
     const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
-    const float *__restrict__ scw, float thresh, unsigned char *__restrict__ pre,
-    QEntry *__restrict__ q, unsigned int *__restrict__ qcount, unsigned int *__restrict__ need,
-    unsigned long long *__restrict__ stats) {
+    const float *__restrict__ scw, float thresh, unsigned char *__restrict__ carry,
+    unsigned char *__restrict__ pre, QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
+    unsigned int *__restrict__ need, int *__restrict__ slist, unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int n_pre = 0;
   if (i < n) {
-    const int s = tp_sys[i];
-    const double *prm = prm_all + (size_t)s * prm_size(NM);
-    const double *xsys = xs0 + (size_t)s * 3 * NM;
-    const double *r0 = rec + (size_t)s * C * REC;
-    const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
-    const double T = sc.h * (double)C;
-    float best = -1e30f;
-    for (int j = 0; j < NM; ++j) {
-      float f[SC_IN];
-      scorer_features(x, y, z, vx, vy, vz, xsys[3 * j], xsys[3 * j + 1], xsys[3 * j + 2],
-                      r0[R_W(NM) + 3 * j], r0[R_W(NM) + 3 * j + 1], r0[R_W(NM) + 3 * j + 2],
-                      sqrt(prm[P_RC2(NM) + j]), T, f);
-      best = fmaxf(best, scorer_fwd(scw, f));
+    bool go = carry[i] != 0;
+    if (go) carry[i] = 0;
+    int s = -1;
+    if (!go && scw != nullptr) {
+      s = tp_sys[i];
+      const double *prm = prm_all + (size_t)s * prm_size(NM);
+      const double *xsys = xs0 + (size_t)s * 3 * NM;
+      const double *r0 = rec + (size_t)s * C * REC;
+      const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
+      const double T = sc.h * (double)C;
+      float best = -1e30f;
+      for (int j = 0; j < NM; ++j) {
+        float f[SC_IN];
+        scorer_features(x, y, z, vx, vy, vz, xsys[3 * j], xsys[3 * j + 1], xsys[3 * j + 2],
+                        r0[R_W(NM) + 3 * j], r0[R_W(NM) + 3 * j + 1], r0[R_W(NM) + 3 * j + 2],
+                        sqrt(prm[P_RC2(NM) + j]), T, f);
+        best = fmaxf(best, scorer_fwd(scw, f));
+      }
+      go = best > thresh;
+      n_pre = go ? 1u : 0u;
     }
-    const bool go = best > thresh;
     pre[i] = go ? 1 : 0;
     if (go) {
+      if (s < 0) s = tp_sys[i];
       const unsigned int slot = atomicAdd(qcount, 1u);
-      q_put(q + slot, (int)i, s, 0, 0, x, y, z, vx, vy, vz);
-      need[s] = 1u;
-      n_pre = 1;
+      QEntry *e = q + slot;
+      e->tp = (int)i;
+      e->sys = s;
+      e->k0 = 0;
+      e->phase = 0;
+      flag_system(need, 1u, s, slist, qcount + 1);
     }
   }
   if (stats) wave_add(stats + S_PREBUCKET, n_pre);
@@ -530,17 +583,21 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
 
 // ---------------------------------------------------------------------------
 // BS queue consumer.  The queue is short (tens to thousands of entries) and
-// each entry is a sequential chain of up to C steps, so entries are spread
-// over every wave of the grid: wave w takes entries w*L .. w*L+L-1 with
-// L = ceil(count / waves), one per lane, instead of packing them into a few
-// full waves.  Each entry is carried from its step k0 to the end of the
-// window, with the exact check deciding Kepler or BS at every step.
+// each entry is a sequential chain of up to C steps.  Wave w takes entries
+// w*L .. w*L+L-1, one per lane.  Alone on the GPU the entries are spread over
+// every wave of the grid (L = ceil(count / waves)) for the shortest chain
+// latency; beside the main kernel they are packed 64 to a wave (dense), so
+// that few SIMDs give up register space to this kernel (322 registers a lane)
+// and the main kernel keeps its occupancy.  Each entry is carried from its
+// step k0 to the end of the window, with the exact check deciding Kepler or
+// BS at every step.
 template <int NM, int GUESS>
 __global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
-    const unsigned int *__restrict__ qbegin, TPArrays tp, const double *__restrict__ prm_all,
-    const double *__restrict__ xs0, const double *__restrict__ rec, const double *__restrict__ pc,
-    int C, StepConst sc, const float *__restrict__ nnw, unsigned int *__restrict__ status,
+    const unsigned int *__restrict__ qbegin, int dense, TPArrays tp,
+    const double *__restrict__ prm_all, const double *__restrict__ xs0,
+    const double *__restrict__ rec, const double *__restrict__ pc, int C, StepConst sc,
+    const float *__restrict__ nnw, unsigned int *__restrict__ status,
     unsigned long long *__restrict__ stats) {
   constexpr int REC = rec_size(NM);
   // BS extrapolation tableau of each lane, lane-interleaved in LDS (one-wave blocks)
@@ -554,14 +611,20 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
   const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned int lane = threadIdx.x & 63u;
-  unsigned int L = (n + nw - 1u) / nw;
+  unsigned int L = dense ? 64u : (n + nw - 1u) / nw;
   L = L < 1u ? 1u : (L > 64u ? 64u : L);
   for (unsigned int e = w * L + lane; lane < L && e < n; e += nw * L) {
     const QEntry en = q[e0 + e];
     const int s = en.sys;
     const double *prm = prm_all + (size_t)s * prm_size(NM);
     const double *rsys = rec + (size_t)s * C * REC;
-    double x = en.s[0], y = en.s[1], z = en.s[2], vx = en.s[3], vy = en.s[4], vz = en.s[5];
+    double x, y, z, vx, vy, vz;
+    if (en.phase == 0) {  // queued from the window start: the state is in the TP arrays
+      x = tp.x[en.tp]; y = tp.y[en.tp]; z = tp.z[en.tp];
+      vx = tp.vx[en.tp]; vy = tp.vy[en.tp]; vz = tp.vz[en.tp];
+    } else {
+      x = en.s[0]; y = en.s[1]; z = en.s[2]; vx = en.s[3]; vy = en.s[4]; vz = en.s[5];
+    }
     double ax, ay, az;
     uint32_t st = 0;
     int k = en.k0;

@@ -20,20 +20,23 @@ struct Launchers {
                         const Tile *tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
                         const double *prm, const double *xs0, const double *rec, int C,
                         StepConst sc, const float *nnw, const unsigned char *pre,
-                        QEntry *q, unsigned int *qcount, unsigned int *need,
-                        unsigned int *status, unsigned long long *stats);
+                        QEntry *q, unsigned int *qcount, unsigned int *need, int *slist,
+                        unsigned char *carry, unsigned int *status,
+                        unsigned long long *stats);
   hipError_t (*tp_encounter)(hipStream_t st, int guess, int grid, int block, const QEntry *q,
-                             const unsigned int *qcount, const unsigned int *qbegin, TPArrays tp,
+                             const unsigned int *qcount, const unsigned int *qbegin, int dense,
+                             TPArrays tp,
                              const double *prm, const double *xs0, const double *rec,
                              const double *pc, int C, StepConst sc, const float *nnw,
                              unsigned int *status, unsigned long long *stats);
   hipError_t (*tp_prebucket)(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
                              const double *prm, const double *xs0, const double *rec, int C,
-                             StepConst sc, const float *scw, float thresh, unsigned char *pre,
-                             QEntry *q, unsigned int *qcount, unsigned int *need,
+                             StepConst sc, const float *scw, float thresh,
+                             unsigned char *carry, unsigned char *pre, QEntry *q,
+                             unsigned int *qcount, unsigned int *need, int *slist,
                              unsigned long long *stats);
-  hipError_t (*pcache)(hipStream_t st, int s_begin, int s_end, int C, const double *rec,
-                       const unsigned int *need, unsigned int want, double *pc, StepConst sc);
+  hipError_t (*pcache)(hipStream_t st, const int *list, const unsigned int *count, int C,
+                       const double *rec, double *pc, StepConst sc);
 };
 
 const Launchers *launchers_nm1();

@@ -52,10 +52,11 @@ hipError_t main_pg(hipStream_t st, int block, int nblocks, const Tile *tiles, in
                    int tiles_per_xcd, TPArrays tp, const double *prm, const double *xs0,
                    const double *rec, int C, StepConst sc, const float *nnw,
                    const unsigned char *pre, QEntry *q, unsigned int *qcount,
-                   unsigned int *need, unsigned int *status, unsigned long long *stats) {
+                   unsigned int *need, int *slist, unsigned char *carry, unsigned int *status,
+                   unsigned long long *stats) {
   tp_main_kernel<NMV, P, G><<<nblocks, block, 0, st>>>(tiles, ntiles, tiles_per_xcd, tp, prm, xs0,
                                                        rec, C, sc, nnw, pre, q, qcount, need,
-                                                       status, stats);
+                                                       slist, carry, status, stats);
   return hipGetLastError();
 }
 
@@ -63,12 +64,13 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
                      int ntiles, int tiles_per_xcd, TPArrays tp, const double *prm,
                      const double *xs0, const double *rec, int C, StepConst sc, const float *nnw,
                      const unsigned char *pre, QEntry *q, unsigned int *qcount,
-                     unsigned int *need, unsigned int *status, unsigned long long *stats) {
+                     unsigned int *need, int *slist, unsigned char *carry, unsigned int *status,
+                     unsigned long long *stats) {
   if (block <= 0 || block > 512 || (block & 63)) return hipErrorInvalidValue;
 #define MINT_MAIN(PP, GG)                                                                       \
   if (P == PP && guess == GG)                                                                   \
     return main_pg<PP, GG>(st, block, nblocks, tiles, ntiles, tiles_per_xcd, tp, prm, xs0, rec, \
-                           C, sc, nnw, pre, q, qcount, need, status, stats);
+                           C, sc, nnw, pre, q, qcount, need, slist, carry, status, stats);
   MINT_MAIN(1, 0) MINT_MAIN(1, 1) MINT_MAIN(1, 2)
   MINT_MAIN(2, 0) MINT_MAIN(2, 1) MINT_MAIN(2, 2)
   MINT_MAIN(4, 0) MINT_MAIN(4, 1) MINT_MAIN(4, 2)
@@ -77,22 +79,23 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
 }
 
 hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
-                          const unsigned int *qcount, const unsigned int *qbegin, TPArrays tp,
+                          const unsigned int *qcount, const unsigned int *qbegin, int dense,
+                          TPArrays tp,
                           const double *prm, const double *xs0, const double *rec,
                           const double *pc, int C, StepConst sc, const float *nnw,
                           unsigned int *status, unsigned long long *stats) {
   if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
   switch (guess) {
     case 0:
-      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     case 1:
-      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     case 2:
-      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qbegin, tp, prm, xs0, rec, pc,
+      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
                                                           C, sc, nnw, status, stats);
       break;
     default:
@@ -103,21 +106,19 @@ hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const 
 
 hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
                           const double *prm, const double *xs0, const double *rec, int C,
-                          StepConst sc, const float *scw, float thresh, unsigned char *pre,
-                          QEntry *q, unsigned int *qcount, unsigned int *need,
-                          unsigned long long *stats) {
+                          StepConst sc, const float *scw, float thresh, unsigned char *carry,
+                          unsigned char *pre, QEntry *q, unsigned int *qcount, unsigned int *need,
+                          int *slist, unsigned long long *stats) {
   if (n <= 0) return hipSuccess;
   tp_prebucket_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, pre, q, qcount, need, stats);
+      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, carry, pre, q, qcount, need, slist, stats);
   return hipGetLastError();
 }
 
-hipError_t pcache_l(hipStream_t st, int s_begin, int s_end, int C, const double *rec,
-                    const unsigned int *need, unsigned int want, double *pc, StepConst sc) {
-  const long long n = (long long)(s_end - s_begin) * C * PC_PTS * NMV;
-  if (n <= 0) return hipSuccess;
-  pcache_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(s_begin, s_end, C, rec, need, want, pc,
-                                                                  sc);
+hipError_t pcache_l(hipStream_t st, const int *list, const unsigned int *count, int C,
+                    const double *rec, double *pc, StepConst sc) {
+  // fixed grid, grid-stride over the listed systems' work items
+  pcache_kernel<NMV><<<1024, 256, 0, st>>>(list, count, C, rec, pc, sc);
   return hipGetLastError();
 }
 

@@ -141,13 +141,13 @@ struct Net<0> {
     for (int i = 0; i < KM_H; ++i) {
       float a = w[KM_B1 + i];
       for (int k = 0; k < KM_IN; ++k) a = fmaf(w[KM_W1 + i * KM_IN + k], in[k], a);
-      h1[i] = tanhf(a);
+      h1[i] = nn_tanh(a);
     }
     float out = w[KM_B3];
     for (int i = 0; i < KM_H; ++i) {
       float a = w[KM_B2 + i];
       for (int k = 0; k < KM_H; ++k) a = fmaf(w[KM_W2 + i * KM_H + k], h1[k], a);
-      h2[i] = tanhf(a);
+      h2[i] = nn_tanh(a);
       out = fmaf(w[KM_W3 + i], h2[i], out);
     }
     const float e = live ? out - t : 0.f;
@@ -187,7 +187,7 @@ struct Net<1> {
     for (int i = 0; i < SC_H; ++i) {
       float a = w[SC_B1 + i];
       for (int k = 0; k < SC_IN; ++k) a = fmaf(w[SC_W1 + i * SC_IN + k], in[k], a);
-      h[i] = tanhf(a);
+      h[i] = nn_tanh(a);
       out = fmaf(w[SC_W2 + i], h[i], out);
     }
     const float p = 1.f / (1.f + expf(-out));

@@ -58,92 +58,32 @@ int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, KTimer *
   return MINT_OK;
 }
 
-hipEvent_t window_event(std::vector<hipEvent_t> &v, size_t i);
-
-// TP side of one launch window of C steps on the context stream: queue reset,
-// then (with the scorer) the pre-bucketing pass whose entries are carried by
-// an encounter kernel on the side stream while the main kernel runs, then the
-// main kernel, the perturber cache of systems that deferred mid-window, and
-// the encounter kernel over those deferrals.
-int window_tp(Ctx *c, const Shape &sh, int C, const double *xs0, const double *rec, KTimer *kt) {
-  const StepConst sc = c->step_const();
-  const TPArrays tp = c->tp_arrays();
-  unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  const float *nnw = c->d_nnw_kep;
-  const float *scw = c->cfg.use_scorer ? c->d_nnw_sc : nullptr;
-  const int tpx = (c->ntiles + 7) / 8;
-  const int nblocks = 8 * tpx;
-  const int guess = c->cfg.kepler_guess;
-  int rc;
-  zero_u32_kernel<<<1, 64, 0, c->stream>>>(c->d_qctl, 4);
-  zero_u32_kernel<<<(c->S + 255) / 256, 256, 0, c->stream>>>(c->d_need, c->S);
-  if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
-  const bool side = scw && c->Ntp > 0;
-  if (side) {
-    rc = c->hip(c->L->tp_prebucket(c->stream, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs0, rec, C, sc, scw,
-                                   c->score_thresh, c->d_pre, c->d_q, c->d_qctl, c->d_need, stats),
-                "prebucket launch");
+// Event i of a per-window event list (created on first use).
+int window_event(Ctx *c, std::vector<hipEvent_t> &v, size_t i, hipEvent_t *out) {
+  while (v.size() <= i) {
+    hipEvent_t e = nullptr;
+    int rc = c->hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
     if (rc) return rc;
-    copy_u32_kernel<<<1, 1, 0, c->stream>>>(c->d_qctl + 2, c->d_qctl);
-    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, 1u, c->d_pc, sc), "perturber cache launch");
-    if (rc) return rc;
-    if (kt) {
-      kt->start();
-      rc = c->hip(c->L->tp_encounter(c->stream, guess, enc_grid(c), 64, c->d_q, c->d_qctl + 2, nullptr, tp,
-                                     c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status, stats),
-                  "encounter launch");
-      if (rc) return rc;
-      kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
-    } else {
-      hipEvent_t f = window_event(c->ev_side, 0);
-      if ((rc = c->hip(hipEventRecord(f, c->stream), "side fork"))) return rc;
-      if ((rc = c->hip(hipStreamWaitEvent(c->stream3, f, 0), "side wait"))) return rc;
-      rc = c->hip(c->L->tp_encounter(c->stream3, guess, enc_grid(c), 64, c->d_q, c->d_qctl + 2, nullptr, tp,
-                                     c->d_prm, xs0, rec, c->d_pc, C, sc, nnw, c->d_status, stats),
-                  "encounter launch");
-      if (rc) return rc;
-    }
+    v.push_back(e);
   }
-  if (c->ntiles > 0) {
-    if (kt) kt->start();
-    rc = c->hip(c->L->tp_main(c->stream, guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx, tp, c->d_prm,
-                              xs0, rec, C, sc, nnw, scw ? c->d_pre : nullptr, c->d_q, c->d_qctl, c->d_need,
-                              c->d_status, stats),
-                "main launch");
-    if (rc) return rc;
-    if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
-    if (kt) kt->start();
-    rc = c->hip(c->L->pcache(c->stream, 0, c->S, C, rec, c->d_need, 2u, c->d_pc, sc), "perturber cache launch");
-    if (rc) return rc;
-    rc = c->hip(c->L->tp_encounter(c->stream, guess, enc_grid(c), 64, c->d_q, c->d_qctl,
-                                   side ? c->d_qctl + 2 : nullptr, tp, c->d_prm, xs0, rec, c->d_pc, C, sc, nnw,
-                                   c->d_status, stats),
-                "encounter launch");
-    if (rc) return rc;
-    if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
-  }
-  if (side && !kt) {
-    hipEvent_t j = window_event(c->ev_side, 1);
-    if ((rc = c->hip(hipEventRecord(j, c->stream3), "side join"))) return rc;
-    if ((rc = c->hip(hipStreamWaitEvent(c->stream, j, 0), "side join wait"))) return rc;
-  }
+  *out = v[i];
   return MINT_OK;
 }
 
-hipEvent_t window_event(std::vector<hipEvent_t> &v, size_t i) {
-  while (v.size() <= i) {
-    hipEvent_t e = nullptr;
-    hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    v.push_back(e);
-  }
-  return v[i];
-}
-
-// Enqueue nsteps of the batched form.  Pipelined: the ephemeris of window
-// w+1 runs on the second stream while the TP kernels of window w run on the
-// context stream, with the two ephemeris buffers alternating; window w's TP
-// kernels wait for its ephemeris, and the ephemeris of window w+2 waits for
-// window w's TP kernels (it overwrites their buffer).
+// Enqueue nsteps of the batched form as launch windows of up to sh.C steps.
+//
+// Window w, on the context stream: reset of the queue of parity w%2, the
+// queue-head pass (TPs carried from window w-1, plus the scorer's picks),
+// then the main kernel, whose mid-window deferrals are queued for their tail
+// and carried into window w+1.  On the queue stream, forked after the head
+// pass: the tails of window w-1 (perturber cache, encounter kernel), then the
+// head of window w (cache, encounter kernel); joined back before window w+1.
+// A tail thus runs beside the next main kernel instead of after its own.  The
+// ephemeris stream computes window w into buffer w%3 while window w-1 runs;
+// that buffer was last read by window w-3 and by its tails, served in window
+// w-2.  The last window's tails run on the context stream at the end.
+// Without pipelining (or when timing each kernel) the same launches run in
+// the same order on the context stream.
 int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTimer *kt) {
   std::vector<int> win;
   for (int64_t done = 0; done < nsteps;) {
@@ -151,31 +91,98 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     win.push_back(C);
     done += C;
   }
-  double *xs[2] = {c->d_xs0, c->d_xs0b};
-  double *rec[2] = {c->d_rec, c->d_recb};
+  const int W = (int)win.size();
+  const StepConst sc = c->step_const();
+  const TPArrays tp = c->tp_arrays();
+  unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
+  const float *nnw = c->d_nnw_kep;
+  const float *scw = c->cfg.use_scorer ? c->d_nnw_sc : nullptr;
+  const int guess = c->cfg.kepler_guess;
+  const int tpx = (c->ntiles + 7) / 8;
+  const int nblocks = 8 * tpx;
+  const int S = c->S;
+  double *xs[3] = {c->d_xs0, c->d_xs0b, c->d_xs0c};
+  double *rec[3] = {c->d_rec, c->d_recb, c->d_recc};
+  QEntry *q[2] = {c->d_q, c->d_q2};
+  unsigned int *ctl[2] = {c->d_qctl, c->d_qctl + 4};
+  unsigned int *need[2] = {c->d_need, c->d_need + S};
+  int *slist_head[2] = {c->d_slist, c->d_slist + 2 * S};
+  int *slist_tail[2] = {c->d_slist + S, c->d_slist + 3 * S};
+  const bool pipe = pipelined && !kt;
+  hipStream_t sm = c->stream;
+  hipStream_t sp = pipe ? c->stream2 : sm;  // ephemeris
+  hipStream_t sq = pipe ? c->stream3 : sm;  // queue work
   int rc;
-  if (!pipelined || win.size() < 2) {
-    for (int C : win) {
-      if ((rc = window_eph(c, c->stream, C, xs[0], rec[0], kt))) return rc;
-      if ((rc = window_tp(c, sh, C, xs[0], rec[0], kt))) return rc;
-    }
+  auto record = [&](std::vector<hipEvent_t> &v, size_t i, hipStream_t st) -> int {
+    if (!pipe) return MINT_OK;
+    hipEvent_t e;
+    int r = window_event(c, v, i, &e);
+    return r ? r : c->hip(hipEventRecord(e, st), "event record");
+  };
+  auto wait = [&](hipStream_t st, std::vector<hipEvent_t> &v, size_t i) -> int {
+    if (!pipe) return MINT_OK;
+    hipEvent_t e;
+    int r = window_event(c, v, i, &e);
+    return r ? r : c->hip(hipStreamWaitEvent(st, e, 0), "event wait");
+  };
+  // encounter work of window u on stream st: the queue head (entries
+  // [0, head count), systems flagged 1) or the tails (entries after the head,
+  // systems flagged 2; those flagged 1 are cached already).  Beside the main
+  // kernel the entries are packed densely into fewer waves.
+  auto serve = [&](int u, bool tail, hipStream_t st) -> int {
+    const int dense = st != sm ? 1 : 0;
+    const int pu = u % 2, bu = u % 3;
+    if (kt) kt->start();
+    int r = c->hip(c->L->pcache(st, tail ? slist_tail[pu] : slist_head[pu], tail ? ctl[pu] + 3 : ctl[pu] + 1,
+                                win[u], rec[bu], c->d_pc, sc),
+                   "perturber cache launch");
+    if (r) return r;
+    const int grid = dense ? std::min(enc_grid(c), 256) : enc_grid(c);
+    r = c->hip(c->L->tp_encounter(st, guess, grid, 64, q[pu], tail ? ctl[pu] : ctl[pu] + 2,
+                                  tail ? ctl[pu] + 2 : nullptr, dense, tp, c->d_prm, xs[bu], rec[bu], c->d_pc,
+                                  win[u], sc, nnw, c->d_status, stats),
+               "encounter launch");
+    if (r) return r;
+    if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
     return MINT_OK;
-  }
-  hipEvent_t fork = window_event(c->ev_eph, 0);
-  if ((rc = c->hip(hipEventRecord(fork, c->stream), "fork event"))) return rc;
-  if ((rc = c->hip(hipStreamWaitEvent(c->stream2, fork, 0), "fork wait"))) return rc;
-  for (size_t w = 0; w < win.size(); ++w) {
-    const int b = (int)(w & 1);
-    if (w >= 2 && (rc = c->hip(hipStreamWaitEvent(c->stream2, window_event(c->ev_tp, w - 2), 0), "buffer wait")))
+  };
+  if ((rc = record(c->ev_side, 0, sm)) || (rc = wait(sp, c->ev_side, 0))) return rc;
+  for (int w = 0; w < W; ++w) {
+    const int p = w % 2, B = w % 3, C = win[w];
+    // ephemeris of window w
+    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
+    if ((rc = window_eph(c, sp, C, xs[B], rec[B], kt))) return rc;
+    if ((rc = record(c->ev_eph, w, sp)) || (rc = wait(sm, c->ev_eph, w))) return rc;
+    // queue head of window w
+    zero_u32_kernel<<<1, 64, 0, sm>>>(ctl[p], 4);
+    zero_u32_kernel<<<(S + 255) / 256, 256, 0, sm>>>(need[p], S);
+    if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
+    if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs[B], rec[B], C, sc, scw,
+                                        c->score_thresh, c->d_carry, c->d_pre, q[p], ctl[p], need[p],
+                                        slist_head[p], stats),
+                     "queue head launch")))
       return rc;
-    if ((rc = window_eph(c, c->stream2, win[w], xs[b], rec[b], nullptr))) return rc;
-    hipEvent_t e = window_event(c->ev_eph, w + 1);
-    if ((rc = c->hip(hipEventRecord(e, c->stream2), "ephemeris event"))) return rc;
-    if ((rc = c->hip(hipStreamWaitEvent(c->stream, e, 0), "ephemeris wait"))) return rc;
-    if ((rc = window_tp(c, sh, win[w], xs[b], rec[b], nullptr))) return rc;
-    if ((rc = c->hip(hipEventRecord(window_event(c->ev_tp, w), c->stream), "window event"))) return rc;
+    copy_u32_kernel<<<1, 1, 0, sm>>>(ctl[p] + 2, ctl[p]);
+    if ((rc = c->hip(hipGetLastError(), "queue head count launch"))) return rc;
+    // queue stream: tails of window w-1, then the head of window w
+    if ((rc = record(c->ev_fork, w, sm)) || (rc = wait(sq, c->ev_fork, w))) return rc;
+    if (w >= 1 && (rc = serve(w - 1, true, sq))) return rc;
+    if ((rc = serve(w, false, sq))) return rc;
+    if ((rc = record(c->ev_join, w, sq))) return rc;
+    // main kernel of window w; its deferrals are carried unless it is the last
+    if (c->ntiles > 0) {
+      if (kt) kt->start();
+      if ((rc = c->hip(c->L->tp_main(sm, guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx, tp,
+                                     c->d_prm, xs[B], rec[B], C, sc, nnw, c->d_pre, q[p], ctl[p], need[p],
+                                     slist_tail[p], w + 1 < W ? c->d_carry : nullptr, c->d_status, stats),
+                       "main launch")))
+        return rc;
+      if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
+    }
+    if ((rc = wait(sm, c->ev_join, w)) || (rc = record(c->ev_tp, w, sm))) return rc;
   }
-  return MINT_OK;
+  // the last window's tails
+  return serve(W - 1, true, sm);
 }
 
 int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out) {
@@ -324,8 +331,10 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   hipDeviceProp_t prop;
   hipGetDeviceProperties(&prop, c->cfg.device);
   char key[512];
-  std::snprintf(key, sizeof(key), "%s|%s|nm=%d|S=%d|N=%lld|dt=%.17g|guess=%d|scorer=%d", prop.gcnArchName,
-                prop.name, c->NM, c->S, c->Ntp, c->cfg.dt, c->cfg.kepler_guess, c->cfg.use_scorer);
+  std::snprintf(key, sizeof(key), "%s|%s|nm=%d|S=%d|N=%lld|dt=%.17g|guess=%d|scorer=%d|fixed=%d,%d,%d",
+                prop.gcnArchName, prop.name, c->NM, c->S, c->Ntp, c->cfg.dt, c->cfg.kepler_guess,
+                c->cfg.use_scorer, std::max(0, c->cfg.block_size), std::max(0, c->cfg.tps_per_lane),
+                std::max(0, c->cfg.chunk_steps));
   for (char *p = key; *p; ++p)
     if (*p == ' ') *p = '_';
   if (cache_path) {
@@ -377,12 +386,15 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
     for (int d = 0; d < 6; ++d) hipMemcpy(c->d_tp[d], stp[d], tb, hipMemcpyDeviceToDevice);
   };
   const mint_config saved = c->cfg;
-  const int blocks[] = {64, 128, 256, 512};
-  const int Ps[] = {1, 2, 4};
-  const int Cs[] = {32, 64, 128};
-  const int probe = 128;
+  // candidates; a parameter set in the configuration is held
+  std::vector<int> blocks = {64, 128, 256, 512}, Ps = {1, 2, 4}, Cs = {32, 64, 128};
+  if (saved.block_size > 0) blocks = {saved.block_size};
+  if (saved.tps_per_lane > 0) Ps = {saved.tps_per_lane};
+  if (saved.chunk_steps > 0) Cs = {saved.chunk_steps};
+  // two or more windows, so the probe includes the pipeline's steady state
+  const int probe = 256;
   double best_ms = 1e300;
-  Shape bs{DEF_BLOCK, DEF_P, DEF_C};
+  Shape bs{blocks[0], Ps[0], Cs[0]};
   int rc = MINT_OK;
   c->cfg.form = MINT_FORM_BATCHED;
   c->cfg.collect_stats = 0;
@@ -392,6 +404,9 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
         c->cfg.block_size = b;
         c->cfg.tps_per_lane = p;
         c->cfg.chunk_steps = C;
+        // untimed first run (buffers, tiles and the graph for this shape), then the probe
+        restore();
+        if ((rc = step_impl(c, probe))) break;
         restore();
         hipDeviceSynchronize();
         hipEventRecord(c->ev0, c->stream);
