@@ -415,6 +415,35 @@ __device__ __forceinline__ void scorer_features(double x, double y, double z, do
   f[5] = fminf(rt * iR, 8.0f) * 0.25f;
 }
 
+// Largest scorer logit over the NM pairs of one TP, f[j] the features of
+// pair j.  Hidden unit outermost: each unit's 8 weights are read once (scalar
+// loads) and applied to every pair, so the weights never pile up in scalar
+// registers.  Per pair the sums run in the order of scorer_fwd.
+template <int NM>
+__device__ __forceinline__ float scorer_best(const float (*f)[SC_IN], const float *__restrict__ w) {
+  float out[NM];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) out[j] = w[SC_B2];
+#pragma unroll 2
+  for (int i = 0; i < SC_H; ++i) {
+    const float b1 = w[SC_B1 + i], w2 = w[SC_W2 + i];
+    float w1[SC_IN];
+#pragma unroll
+    for (int k = 0; k < SC_IN; ++k) w1[k] = w[SC_W1 + i * SC_IN + k];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      float a = b1;
+#pragma unroll
+      for (int k = 0; k < SC_IN; ++k) a = fmaf(w1[k], f[j][k], a);
+      out[j] = fmaf(w2, nn_tanh(a), out[j]);
+    }
+  }
+  float best = -1e30f;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) best = fmaxf(best, out[j]);
+  return best;
+}
+
 template <int NM, int P, int GUESS>
 __global__ void __launch_bounds__(512) tp_main_kernel(
     const Tile *__restrict__ tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
@@ -555,15 +584,13 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
       const double *r0 = rec + (size_t)s * C * REC;
       const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
       const double T = sc.h * (double)C;
-      float best = -1e30f;
-      for (int j = 0; j < NM; ++j) {
-        float f[SC_IN];
+      float f[NM][SC_IN];
+#pragma unroll
+      for (int j = 0; j < NM; ++j)
         scorer_features(x, y, z, vx, vy, vz, xsys[3 * j], xsys[3 * j + 1], xsys[3 * j + 2],
                         r0[R_W(NM) + 3 * j], r0[R_W(NM) + 3 * j + 1], r0[R_W(NM) + 3 * j + 2],
-                        sqrt(prm[P_RC2(NM) + j]), T, f);
-        best = fmaxf(best, scorer_fwd(scw, f));
-      }
-      go = best > thresh;
+                        sqrt(prm[P_RC2(NM) + j]), T, f[j]);
+      go = scorer_best<NM>(f, scw) > thresh;
       n_pre = go ? 1u : 0u;
     }
     pre[i] = go ? 1 : 0;
