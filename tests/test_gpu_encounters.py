@@ -70,3 +70,26 @@ def test_graph_replay_equals_direct_launches():
     b, _, _ = run_gpu(st, 70, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=16, block_size=128,
                       tps_per_lane=1, use_graph=0)
     assert np.array_equal(a.tx, b.tx) and np.array_equal(a.tv, b.tv)
+
+
+def test_timed_serial_launches_and_graph_reuse_match():
+    """The pipelined graph, the serial launches used when every kernel is
+    timed, and two calls replaying one cached graph (TPs deferred in the last
+    window of a call are finished within that call, none is carried across)
+    give the same bits and the same work counts."""
+    st = encounter_system(n_tp=256, seed=5)
+    kw = dict(chunk_steps=16, block_size=64, tps_per_lane=1)
+    print("encounter cloud: graph / timed serial / two calls, 96 steps", flush=True)
+    a, _, sa = run_gpu(st, 96, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, **kw)
+    b, _, sb = run_gpu(st, 96, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, stats=3, **kw)
+    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES, collect_stats=1, **kw) as g:
+        g.upload(st)
+        g.step(48)
+        g.step(48)
+        c = g.download()
+        sc = g.stats()
+    for o in (b, c):
+        assert np.array_equal(a.tx, o.tx) and np.array_equal(a.tv, o.tv) and np.array_equal(a.mx, o.mx)
+    assert sa["bs_steps"] > 0 and sa["deferred"] > 0
+    for key in ("tp_steps", "bs_steps", "bs_rhs_evals", "kepler_solves"):
+        assert sa[key] == sb[key] == sc[key], key
