@@ -444,8 +444,12 @@ __device__ __forceinline__ float scorer_best(const float (*f)[SC_IN], const floa
   return best;
 }
 
+// One TP per lane: registers are capped for four waves per SIMD (128 VGPRs,
+// ~30 spilled to scratch), which measures 2-3 % faster than three waves with
+// no spills (five waves spill ~65 and run 40 % slower).  Two or four TPs per
+// lane keep the compiler's allocation.
 template <int NM, int P, int GUESS>
-__global__ void __launch_bounds__(512) tp_main_kernel(
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 : 1))) tp_main_kernel(
     const Tile *__restrict__ tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
     const double *__restrict__ prm_all, const double *__restrict__ xs0,
     const double *__restrict__ rec, int C, StepConst sc, const float *__restrict__ nnw,
@@ -481,7 +485,8 @@ __global__ void __launch_bounds__(512) tp_main_kernel(
     }
   }
   if (pre != nullptr) {
-    // TPs the scorer pre-bucketed for this window start in the encounter queue
+    // TPs at the head of this window's queue (carried or picked by the scorer)
+    // are the encounter kernel's for the whole window
 #pragma unroll
     for (int p = 0; p < P; ++p)
       if (act[p] && pre[idx[p]]) {
