@@ -112,9 +112,16 @@ def cpu_baseline(st_full, budget_s=12.0):
     t0 = time.perf_counter()
     _, _, stats = O.run(sample, steps, 0.5, guess=1, threads=1)
     dt = time.perf_counter() - t0
+    # the same sample on every core of this process's share (systems are
+    # independent; the oracle splits them over std::threads)
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    _, _, stats_n = O.run(sample, steps, 0.5, guess=1, threads=ncores)
+    dtn = time.perf_counter() - t0
     return dict(value=stats["tp_steps"] / dt, unit="TP-steps/s", cores=1, kind="port",
                 sample=f"systems 0-{nsys - 1} of the sweep ({sample.n_tp} TPs), {steps} steps, oracle single thread, "
-                       f"{dt:.1f} s")
+                       f"{dt:.1f} s",
+                all_cores=dict(value=stats_n["tp_steps"] / dtn, cores=ncores, seconds=round(dtn, 2)))
 
 
 def copy_probe(nbytes):
