@@ -377,12 +377,73 @@ __device__ __forceinline__ bool encounter_tp(double x, double y, double z,
 // the drift: Kepler force of the (unsoftened) Sun plus (1-K)-weighted pulls of
 // the massive bodies, each on its own Kepler orbit from (Y_j, W_j).
 // (update_my_velocity_total, src/sim.cuh:410-443, with moving perturbers.)
+//
+// G > 1: G lanes evaluate the right-hand side of one TP together (lanes
+// base .. base+G-1 of the wave, this lane is base+jl, all holding the same
+// TP state): lane jl places perturber jl and forms its pull, then every lane
+// adds the NM pulls in body order from shuffles, so each lane gets the value
+// the serial loop (G = 1) computes, bit for bit.
+// Index 1..PC_PTS of the perturber-cache point at time tau into the step, or
+// 0 when tau is not exactly one of them.
+__device__ __forceinline__ int cache_point(double tau, double h) {
+  const double hs16 = h / (double)PC_PTS;
+  int ci = (int)(tau * ((double)PC_PTS / h) + 0.5);
+  if (!(ci >= 1 && ci <= PC_PTS && (double)ci * hs16 == tau)) ci = 0;
+  return ci;
+}
+
+// Position of perturber j at time tau into the drift: the cache point when
+// there is one, else its Kepler drift from (Y_j, W_j).
 template <int NM>
-__device__ __forceinline__ void rhs_tp(double tau, const double *yv, const double *__restrict__ Y,
-                                       const double *__restrict__ W,
-                                       const double *__restrict__ P4, double h,
-                                       const double *__restrict__ prm, double mu, double eps2,
-                                       double *f) {
+__device__ __forceinline__ void perturber_pos(int j, double tau, int ci, const double *__restrict__ Y,
+                                              const double *__restrict__ W,
+                                              const double *__restrict__ P4, double mu, double &px,
+                                              double &py, double &pz) {
+  px = Y[3 * j + 0];
+  py = Y[3 * j + 1];
+  pz = Y[3 * j + 2];
+  if (ci > 0 && P4 != nullptr) {
+    px = P4[(ci - 1) * 3 * NM + 3 * j + 0];
+    py = P4[(ci - 1) * 3 * NM + 3 * j + 1];
+    pz = P4[(ci - 1) * 3 * NM + 3 * j + 2];
+  } else if (tau != 0.0) {
+    double qx = W[3 * j + 0], qy = W[3 * j + 1], qz = W[3 * j + 2];
+    bool ok;
+    kepler_drift_series(px, py, pz, qx, qy, qz, mu, tau, ok);
+  }
+}
+
+// (1-K)-weighted pull of perturber j at (px, py, pz) on the TP at (x, y, z).
+template <int NM>
+__device__ __forceinline__ void pull_at(int j, double px, double py, double pz, double x, double y,
+                                        double z, const double *__restrict__ prm, double eps2,
+                                        double &w, double &dx, double &dy, double &dz) {
+  dx = px - x;
+  dy = py - y;
+  dz = pz - z;
+  const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+  w = close_weight(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j]);
+}
+
+template <int NM>
+__device__ __forceinline__ void perturber_pull(int j, double tau, double x, double y, double z,
+                                               const double *__restrict__ Y,
+                                               const double *__restrict__ W,
+                                               const double *__restrict__ P4, int ci,
+                                               const double *__restrict__ prm, double mu,
+                                               double eps2, double &w, double &dx, double &dy,
+                                               double &dz) {
+  double px, py, pz;
+  perturber_pos<NM>(j, tau, ci, Y, W, P4, mu, px, py, pz);
+  pull_at<NM>(j, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
+}
+
+// Grouped right-hand side (G lanes per TP, see rhs_tp) with this lane's
+// perturber already placed at (px, py, pz).
+template <int NM>
+__device__ __forceinline__ void rhs_tp_at(const double *yv, double px, double py, double pz,
+                                          const double *__restrict__ prm, double mu, double eps2,
+                                          double *f, int jl, int base) {
   const double x = yv[0], y = yv[1], z = yv[2];
   f[0] = yv[3];
   f[1] = yv[4];
@@ -391,28 +452,54 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
   const double r = sqrt(r2);
   const double w0 = mu / (r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
-  const double hs16 = h / (double)PC_PTS;
-  int ci = (int)(tau * ((double)PC_PTS / h) + 0.5);
-  if (!(ci >= 1 && ci <= PC_PTS && (double)ci * hs16 == tau)) ci = 0;
+  double w, dx, dy, dz;
+  pull_at<NM>(jl < NM ? jl : NM - 1, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
-    double px = Y[3 * j + 0], py = Y[3 * j + 1], pz = Y[3 * j + 2];
-    if (ci > 0 && P4 != nullptr) {
-      px = P4[(ci - 1) * 3 * NM + 3 * j + 0];
-      py = P4[(ci - 1) * 3 * NM + 3 * j + 1];
-      pz = P4[(ci - 1) * 3 * NM + 3 * j + 2];
-    } else if (tau != 0.0) {
-      double qx = W[3 * j + 0], qy = W[3 * j + 1], qz = W[3 * j + 2];
-      bool ok;
-      kepler_drift_series(px, py, pz, qx, qy, qz, mu, tau, ok);
+    const double wj = __shfl(w, base + j, 64);
+    ax = fma(wj, __shfl(dx, base + j, 64), ax);
+    ay = fma(wj, __shfl(dy, base + j, 64), ay);
+    az = fma(wj, __shfl(dz, base + j, 64), az);
+  }
+  f[3] = ax;
+  f[4] = ay;
+  f[5] = az;
+}
+
+template <int NM, int G = 1>
+__device__ __forceinline__ void rhs_tp(double tau, const double *yv, const double *__restrict__ Y,
+                                       const double *__restrict__ W,
+                                       const double *__restrict__ P4, double h,
+                                       const double *__restrict__ prm, double mu, double eps2,
+                                       double *f, int jl = 0, int base = 0) {
+  const double x = yv[0], y = yv[1], z = yv[2];
+  f[0] = yv[3];
+  f[1] = yv[4];
+  f[2] = yv[5];
+  const double r2 = dot3(x, y, z, x, y, z);
+  const double r = sqrt(r2);
+  const double w0 = mu / (r2 * r);
+  double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
+  const int ci = cache_point(tau, h);
+  if (G == 1) {
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      double w, dx, dy, dz;
+      perturber_pull<NM>(j, tau, x, y, z, Y, W, P4, ci, prm, mu, eps2, w, dx, dy, dz);
+      ax = fma(w, dx, ax);
+      ay = fma(w, dy, ay);
+      az = fma(w, dz, az);
     }
-    const double dx = px - x, dy = py - y, dz = pz - z;
-    const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double w = close_weight(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j],
-                                  prm[P_RCWINV(NM) + j]);
-    ax = fma(w, dx, ax);
-    ay = fma(w, dy, ay);
-    az = fma(w, dz, az);
+  } else {
+    double w, dx, dy, dz;
+    perturber_pull<NM>(jl < NM ? jl : NM - 1, tau, x, y, z, Y, W, P4, ci, prm, mu, eps2, w, dx, dy, dz);
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      const double wj = __shfl(w, base + j, 64);
+      ax = fma(wj, __shfl(dx, base + j, 64), ax);
+      ay = fma(wj, __shfl(dy, base + j, 64), ay);
+      az = fma(wj, __shfl(dz, base + j, 64), az);
+    }
   }
   f[3] = ax;
   f[4] = ay;
@@ -423,13 +510,13 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
 // (Hairer, Norsett & Wanner, Solving ODEs I, §II.9), on y = (x, v).
 // Replaces modified_midpoint (src/sim.cuh:445-500), whose half-updated
 // velocity does not have an even error expansion.
-template <int NM>
+template <int NM, int G = 1>
 __device__ __forceinline__ void gragg_tp(const double *y0, const double *f0, double t0, double H,
                                       int N, const double *__restrict__ Y,
                                       const double *__restrict__ W,
                                       const double *__restrict__ P4, double h,
                                       const double *__restrict__ prm, double mu, double eps2,
-                                      double *out, int &nrhs) {
+                                      double *out, int &nrhs, int jl = 0, int base = 0) {
   const double hs = H / (double)N;
   const double h2 = 2.0 * hs;
   double z0[6], z1[6], f[6];
@@ -438,16 +525,42 @@ __device__ __forceinline__ void gragg_tp(const double *y0, const double *f0, dou
     z0[i] = y0[i];
     z1[i] = fma(hs, f0[i], y0[i]);
   }
-  for (int m = 1; m < N; ++m) {
-    rhs_tp<NM>(t0 + (double)m * hs, z1, Y, W, P4, h, prm, mu, eps2, f);
+  if (G == 1) {
+    for (int m = 1; m < N; ++m) {
+      rhs_tp<NM, G>(t0 + (double)m * hs, z1, Y, W, P4, h, prm, mu, eps2, f, jl, base);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const double z2 = fma(h2, f[i], z0[i]);
-      z0[i] = z1[i];
-      z1[i] = z2;
+      for (int i = 0; i < 6; ++i) {
+        const double z2 = fma(h2, f[i], z0[i]);
+        z0[i] = z1[i];
+        z1[i] = z2;
+      }
     }
+    rhs_tp<NM, G>(t0 + H, z1, Y, W, P4, h, prm, mu, eps2, f, jl, base);
+  } else {
+    // the lane's perturber is placed one substep ahead (a cache load or a
+    // drift, independent of the TP state), so its latency overlaps the
+    // current right-hand side
+    const int j = jl < NM ? jl : NM - 1;
+    double px, py, pz;
+    const double t1 = N > 1 ? t0 + hs : t0 + H;
+    perturber_pos<NM>(j, t1, cache_point(t1, h), Y, W, P4, mu, px, py, pz);
+    for (int m = 1; m < N; ++m) {
+      const double tn = m + 1 < N ? t0 + (double)(m + 1) * hs : t0 + H;
+      double nx, ny, nz;
+      perturber_pos<NM>(j, tn, cache_point(tn, h), Y, W, P4, mu, nx, ny, nz);
+      rhs_tp_at<NM>(z1, px, py, pz, prm, mu, eps2, f, jl, base);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const double z2 = fma(h2, f[i], z0[i]);
+        z0[i] = z1[i];
+        z1[i] = z2;
+      }
+      px = nx;
+      py = ny;
+      pz = nz;
+    }
+    rhs_tp_at<NM>(z1, px, py, pz, prm, mu, eps2, f, jl, base);
   }
-  rhs_tp<NM>(t0 + H, z1, Y, W, P4, h, prm, mu, eps2, f);
   nrhs += N;
 #pragma unroll
   for (int i = 0; i < 6; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
@@ -464,27 +577,28 @@ __device__ __forceinline__ double bs_fac(int j) {
 // 3-row richardson_extrapolation of src/sim.cuh:502-551.  Returns status bits.
 // The extrapolation tableau element (row, i) lives at Rt[(row * 6 + i) * rs]:
 // a private array (rs = 1) or a lane's column of an LDS block (rs = lanes).
-template <int NM>
+// G > 1: the G lanes of a group run the same step together (see rhs_tp).
+template <int NM, int G = 1>
 __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
                                                 const double *__restrict__ W,
                                                 const double *__restrict__ P4,
                                                 const double *__restrict__ prm, double mu,
                                                 double eps2, double tol, int &nrhs, double *Rt,
-                                                int rs) {
+                                                int rs, int jl = 0, int base = 0) {
   uint32_t st = 0;
   double t = 0.0, H = h;
   int depth = 0;
   while (t < h) {
     if (H > h - t) H = h - t;
     double f0[6];
-    rhs_tp<NM>(t, yv, Y, W, P4, h, prm, mu, eps2, f0);
+    rhs_tp<NM, G>(t, yv, Y, W, P4, h, prm, mu, eps2, f0, jl, base);
     nrhs += 1;
     const double xs = sqrt(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
     const double vs = sqrt(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
     double cur[6];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
-      gragg_tp<NM>(yv, f0, t, H, 2 << k, Y, W, P4, h, prm, mu, eps2, cur, nrhs);
+      gragg_tp<NM, G>(yv, f0, t, H, 2 << k, Y, W, P4, h, prm, mu, eps2, cur, nrhs, jl, base);
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
 #pragma unroll

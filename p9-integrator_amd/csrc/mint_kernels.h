@@ -615,15 +615,17 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
 
 // ---------------------------------------------------------------------------
 // BS queue consumer.  The queue is short (tens to thousands of entries) and
-// each entry is a sequential chain of up to C steps.  Wave w takes entries
-// w*L .. w*L+L-1, one per lane.  Alone on the GPU the entries are spread over
-// every wave of the grid (L = ceil(count / waves)) for the shortest chain
-// latency; beside the main kernel they are packed 64 to a wave (dense), so
-// that few SIMDs give up register space to this kernel (322 registers a lane)
-// and the main kernel keeps its occupancy.  Each entry is carried from its
-// step k0 to the end of the window, with the exact check deciding Kepler or
-// BS at every step.
-template <int NM, int GUESS>
+// each entry is a sequential chain of up to C steps.  G lanes serve one entry
+// (G = 1, or G = 8 with each BS right-hand side split over the perturbers,
+// for the shortest chain); wave w takes entries w*L .. w*L+L-1, one per
+// group of G lanes.  Spread, the entries go over every wave of the grid
+// (L = ceil(count / waves)); dense, beside a long main kernel, they are
+// packed 64 / G to a wave, so that few SIMDs give up register space to this
+// kernel (322 registers a lane) and the main kernel keeps its occupancy.
+// Each entry is carried from its step k0 to the end of the window, with the
+// exact check deciding Kepler or BS at every step.  The lanes of a group
+// compute the same values; the group's first lane writes them.
+template <int NM, int GUESS, int G>
 __global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry *__restrict__ q, const unsigned int *__restrict__ qcount,
     const unsigned int *__restrict__ qbegin, int dense, TPArrays tp,
@@ -643,9 +645,12 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
   const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned int lane = threadIdx.x & 63u;
-  unsigned int L = dense ? 64u : (n + nw - 1u) / nw;
-  L = L < 1u ? 1u : (L > 64u ? 64u : L);
-  for (unsigned int e = w * L + lane; lane < L && e < n; e += nw * L) {
+  constexpr unsigned int GPW = 64u / (unsigned int)G;  // groups per wave
+  const unsigned int g = lane / (unsigned int)G, jl = lane % (unsigned int)G, base = lane - jl;
+  const unsigned int lead = jl == 0u ? 1u : 0u;
+  unsigned int L = dense ? GPW : (n + nw - 1u) / nw;
+  L = L < 1u ? 1u : (L > GPW ? GPW : L);
+  for (unsigned int e = w * L + g; g < L && e < n; e += nw * L) {
     const QEntry en = q[e0 + e];
     const int s = en.sys;
     const double *prm = prm_all + (size_t)s * prm_size(NM);
@@ -678,15 +683,15 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
-        st |= bs_drift_tp<NM>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
-                              pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol, nr,
-                              Rt, 64);
+        st |= bs_drift_tp<NM, G>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
+                                 pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol,
+                                 nr, Rt, 64, (int)jl, (int)base);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
-        ++n_bs;
-        n_rhs += (unsigned)nr;
+        n_bs += lead;
+        n_rhs += lead * (unsigned)nr;
       } else {
-        n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
-        ++n_kep;
+        n_newton += lead * (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
+        n_kep += lead;
       }
       x = fma(sc.hh, r[R_Q2(NM) + 0], x);
       y = fma(sc.hh, r[R_Q2(NM) + 1], y);
@@ -695,7 +700,7 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       vx = fma(sc.hh, ax, vx);
       vy = fma(sc.hh, ay, vy);
       vz = fma(sc.hh, az, vz);
-      ++n_step;
+      n_step += lead;
       if (++k >= C) break;
       const double *rn = rsys + (size_t)k * REC;
       vx = fma(sc.hh, ax, vx);
@@ -706,11 +711,13 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       z = fma(sc.hh, rn[R_Q1(NM) + 2], z);
       flag = encounter_tp<NM>(x, y, z, rn + R_Y(NM), prm);
     }
-    const long long i = en.tp;
-    tp.x[i] = x; tp.y[i] = y; tp.z[i] = z;
-    tp.vx[i] = vx; tp.vy[i] = vy; tp.vz[i] = vz;
-    if (!finite6(x, y, z, vx, vy, vz)) st |= ST_NAN;
-    if (st) atomicOr(&status[s], st);
+    if (lead) {
+      const long long i = en.tp;
+      tp.x[i] = x; tp.y[i] = y; tp.z[i] = z;
+      tp.vx[i] = vx; tp.vy[i] = vy; tp.vz[i] = vz;
+      if (!finite6(x, y, z, vx, vy, vz)) st |= ST_NAN;
+      if (st) atomicOr(&status[s], st);
+    }
   }
   if (stats) {
     wave_add(stats + S_TPSTEPS, n_step);

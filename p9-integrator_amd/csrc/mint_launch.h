@@ -25,7 +25,7 @@ struct Launchers {
                         unsigned long long *stats);
   hipError_t (*tp_encounter)(hipStream_t st, int guess, int grid, int block, const QEntry *q,
                              const unsigned int *qcount, const unsigned int *qbegin, int dense,
-                             TPArrays tp,
+                             int grouped, TPArrays tp,
                              const double *prm, const double *xs0, const double *rec,
                              const double *pc, int C, StepConst sc, const float *nnw,
                              unsigned int *status, unsigned long long *stats);

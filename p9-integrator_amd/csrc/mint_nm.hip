@@ -78,30 +78,45 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
   return hipErrorInvalidValue;
 }
 
-hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
-                          const unsigned int *qcount, const unsigned int *qbegin, int dense,
-                          TPArrays tp,
-                          const double *prm, const double *xs0, const double *rec,
-                          const double *pc, int C, StepConst sc, const float *nnw,
-                          unsigned int *status, unsigned long long *stats) {
-  if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
+// lanes per queue entry in the grouped encounter kernel: one per perturber,
+// rounded up to a power of two
+constexpr int ENC_G = NMV <= 1 ? 1 : (NMV <= 2 ? 2 : (NMV <= 4 ? 4 : 8));
+
+template <int G>
+hipError_t encounter_g(hipStream_t st, int guess, int grid, const QEntry *q, const unsigned int *qcount,
+                       const unsigned int *qbegin, int dense, TPArrays tp, const double *prm,
+                       const double *xs0, const double *rec, const double *pc, int C, StepConst sc,
+                       const float *nnw, unsigned int *status, unsigned long long *stats) {
   switch (guess) {
     case 0:
-      tp_encounter_kernel<NMV, 0><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
-                                                          C, sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 0, G><<<grid, 64, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc, C,
+                                                          sc, nnw, status, stats);
       break;
     case 1:
-      tp_encounter_kernel<NMV, 1><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
-                                                          C, sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 1, G><<<grid, 64, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc, C,
+                                                          sc, nnw, status, stats);
       break;
     case 2:
-      tp_encounter_kernel<NMV, 2><<<grid, block, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc,
-                                                          C, sc, nnw, status, stats);
+      tp_encounter_kernel<NMV, 2, G><<<grid, 64, 0, st>>>(q, qcount, qbegin, dense, tp, prm, xs0, rec, pc, C,
+                                                          sc, nnw, status, stats);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const QEntry *q,
+                          const unsigned int *qcount, const unsigned int *qbegin, int dense, int grouped,
+                          TPArrays tp, const double *prm, const double *xs0, const double *rec,
+                          const double *pc, int C, StepConst sc, const float *nnw,
+                          unsigned int *status, unsigned long long *stats) {
+  if (block != 64) return hipErrorInvalidValue;  // one wave per block: the LDS tableau is per wave
+  if (grouped)
+    return encounter_g<ENC_G>(st, guess, grid, q, qcount, qbegin, dense, tp, prm, xs0, rec, pc, C, sc, nnw,
+                              status, stats);
+  return encounter_g<1>(st, guess, grid, q, qcount, qbegin, dense, tp, prm, xs0, rec, pc, C, sc, nnw, status,
+                        stats);
 }
 
 hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,

@@ -127,10 +127,13 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
   };
   // encounter work of window u on stream st: the queue head (entries
   // [0, head count), systems flagged 1) or the tails (entries after the head,
-  // systems flagged 2; those flagged 1 are cached already).  Beside the main
-  // kernel the entries are packed densely into fewer waves.
+  // systems flagged 2; those flagged 1 are cached already).  Beside a main
+  // kernel long enough to hide it (a large batch) the entries are packed one
+  // per lane densely into few waves; otherwise the chains are on the critical
+  // path and each entry gets a group of lanes that splits its BS work.
+  const bool hidden = c->Ntp >= (1LL << 20);
   auto serve = [&](int u, bool tail, hipStream_t st) -> int {
-    const int dense = st != sm ? 1 : 0;
+    const int dense = (st != sm && hidden) ? 1 : 0;
     const int pu = u % 2, bu = u % 3;
     if (kt) kt->start();
     int r = c->hip(c->L->pcache(st, tail ? slist_tail[pu] : slist_head[pu], tail ? ctl[pu] + 3 : ctl[pu] + 1,
@@ -139,7 +142,7 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     if (r) return r;
     const int grid = dense ? std::min(enc_grid(c), 256) : enc_grid(c);
     r = c->hip(c->L->tp_encounter(st, guess, grid, 64, q[pu], tail ? ctl[pu] : ctl[pu] + 2,
-                                  tail ? ctl[pu] + 2 : nullptr, dense, tp, c->d_prm, xs[bu], rec[bu], c->d_pc,
+                                  tail ? ctl[pu] + 2 : nullptr, dense, !dense, tp, c->d_prm, xs[bu], rec[bu], c->d_pc,
                                   win[u], sc, nnw, c->d_status, stats),
                "encounter launch");
     if (r) return r;
