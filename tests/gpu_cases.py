@@ -27,6 +27,29 @@ def encounter_system(n_tp=512, seed=11, sys_index=0):
     return st
 
 
+def planet_cloud(nm, n_tp=96, seed=0):
+    """nm planets (masses 1e-4 (1 + j) Msun, a = 5 x 1.6^j AU, e 0.03, small
+    inclinations) and a cloud of TPs around the first and the last planet
+    (offsets 0.05-1.5 r_c, velocities the planet's own plus a small random
+    part): close encounters at every massive count 1..8."""
+    rng = np.random.default_rng([20261203, 10, nm, seed])
+    m = 1e-4 * (1.0 + np.arange(nm))
+    a = 5.0 * 1.6 ** np.arange(nm)
+    inc = rng.uniform(0.0, 0.02, nm)
+    ang = rng.uniform(0.0, 2 * np.pi, (3, nm))
+    x, v = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], np.full(nm, 0.03), a)
+    rc = mint.hill_radii(1.0, 3.0, m, x, v)
+    tx, tv = [], []
+    for i in range(n_tp):
+        j = 0 if i % 2 == 0 else nm - 1
+        d = rng.normal(size=3)
+        d *= rc[j] * rng.uniform(0.05, 1.5) / np.linalg.norm(d)
+        tx.append(x[j] + d)
+        tv.append(v[j] + rng.normal(size=3) * 0.02 * np.linalg.norm(v[j]))
+    return mint.State(np.array([0, n_tp], np.int64), m[None], x[None], v[None], np.array(tx), np.array(tv),
+                      meta=dict(workload="planet_cloud", nm=nm))
+
+
 def run_gpu(st, nsteps, dt, form, guess, stats=1, **kw):
     with mint.Mint(dt=dt, form=form, kepler_guess=guess, collect_stats=stats, **kw) as g:
         g.upload(st)

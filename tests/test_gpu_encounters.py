@@ -6,7 +6,7 @@ import pytest
 import mint
 import mint_workloads as W
 import oracle as O
-from gpu_cases import encounter_system, run_gpu
+from gpu_cases import encounter_system, planet_cloud, run_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -93,3 +93,18 @@ def test_timed_serial_launches_and_graph_reuse_match():
     assert sa["bs_steps"] > 0 and sa["deferred"] > 0
     for key in ("tp_steps", "bs_steps", "bs_rhs_evals", "kepler_solves"):
         assert sa[key] == sb[key] == sc[key], key
+
+
+@pytest.mark.parametrize("nm", [1, 2, 3, 4, 6, 7, 8])
+def test_encounters_at_every_massive_count(nm):
+    """BS work of TPs beside 1..8 massive bodies (the grouped encounter kernel
+    splits each right-hand side over 1, 2, 4 or 8 lanes) against the oracle."""
+    st = planet_cloud(nm)
+    print(f"planet cloud nm={nm}", flush=True)
+    out, status, s = run_gpu(st, 40, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=16, block_size=64,
+                             tps_per_lane=1)
+    ref, rst, rs = O.run(st, 40, 0.5, guess=1)
+    assert rs["bs_steps"] > 100, rs
+    assert s["bs_steps"] == rs["bs_steps"] and s["bs_rhs_evals"] == rs["bs_rhs_evals"]
+    assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
+    assert np.array_equal(out.mx, ref["mx"]) and np.all(status == rst)
