@@ -199,6 +199,13 @@ int mint_reset_counters(void *ctx);
 /* Change kepler_guess / use_scorer / form between calls without re-uploading. */
 int mint_set_mode(void *ctx, int32_t form, int32_t kepler_guess, int32_t use_scorer);
 
+/* Diagnostic (no counterpart in the reference): evaluate the device's
+ * square root of a squared distance (op 0: out = sqrt(a)) or normal-range
+ * quotient (op 1: out = a / b) on n inputs on device `device`, so callers can
+ * check them against the host's IEEE results.  b may be NULL for op 0. */
+int mint_selftest_arith(int32_t device, int32_t op, int64_t n, const double *a, const double *b,
+                        double *out);
+
 /* Changeover radii r_c = n_hill * a * cbrt(m / (3M)) from heliocentric states
  * (replaces fetch_r_crit, src/sim.cuh:133-164, evaluated once instead of per
  * pair).  a is the osculating semi-major axis; unbound orbits use r. */

@@ -233,6 +233,15 @@ static double dot3h(const double *a, const double *b) {
   return std::fma(a[0], b[0], std::fma(a[1], b[1], a[2] * b[2]));
 }
 
+// device arithmetic self-check (mint_selftest_arith)
+namespace {
+__global__ void selftest_arith_kernel(int op, long long n, const double *__restrict__ a,
+                                      const double *__restrict__ b, double *__restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = op == 0 ? mint::sqrt_dist2(a[i]) : mint::div_normal(a[i], b[i]);
+}
+}  // namespace
+
 extern "C" {
 
 int32_t mint_abi_version(void) { return MINT_ABI_VERSION; }
@@ -321,6 +330,31 @@ const char *mint_last_error(void *ctx) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return g_create_error.c_str();
   return c->err.c_str();
+}
+
+int mint_selftest_arith(int32_t device, int32_t op, int64_t n, const double *a, const double *b,
+                        double *out) {
+  if (n < 0 || !a || !out || (op != 0 && op != 1) || (op == 1 && !b)) return MINT_EINVAL;
+  if (n == 0) return MINT_OK;
+  if (hipSetDevice(device) != hipSuccess) return MINT_EHIP;
+  const size_t bytes = (size_t)n * sizeof(double);
+  double *da = nullptr, *db = nullptr, *dout = nullptr;
+  int rc = MINT_OK;
+  if (hipMalloc(&da, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+      (op == 1 && hipMalloc(&db, bytes) != hipSuccess)) {
+    rc = MINT_ENOMEM;
+  } else if (hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+             (op == 1 && hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)) {
+    rc = MINT_EHIP;
+  } else {
+    selftest_arith_kernel<<<(unsigned)((n + 255) / 256), 256>>>(op, n, da, db, dout);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = MINT_EHIP;
+  }
+  hipFree(da);
+  hipFree(db);
+  hipFree(dout);
+  return rc;
 }
 
 int mint_hill_radii(int64_t n, double M, double n_hill, const double *m, const double *x,

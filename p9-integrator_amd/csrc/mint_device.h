@@ -80,6 +80,40 @@ struct StepConst {
   double tol;   // BS tolerance
 };
 
+// Square root of a squared distance (a positive normal double well above
+// 2^-767, or zero or infinity): the correctly rounded result by the same
+// reciprocal-square-root refinement the compiler uses for sqrt, without its
+// rescaling of inputs below 2^-767 (which a squared distance in AU never is).
+// Zero and infinity are returned as they are.
+__device__ __forceinline__ double sqrt_dist2(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return (x == 0.0 || x == INFINITY) ? x : g;
+}
+
+// Quotient a / b of operands well inside the normal range (|a|, |b| and the
+// quotient between about 2^-900 and 2^900, as every quotient on the step path
+// is): the correctly rounded result by the same Newton-Raphson refinement of
+// the hardware reciprocal the compiler uses for a / b, without its operand
+// rescaling and special-value fix-up, which change nothing there.
+__device__ __forceinline__ double div_normal(double a, double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
+}
+
 __device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by,
                                        double bz) {
   return fma(ax, bx, fma(ay, by, az * bz));
@@ -148,8 +182,8 @@ struct KepStart {
 __device__ __forceinline__ KepStart kep_start(double x, double y, double z, double vx, double vy,
                                               double vz, double mu) {
   KepStart k;
-  k.r0 = sqrt(dot3(x, y, z, x, y, z));
-  k.ir0 = 1.0 / k.r0;
+  k.r0 = sqrt_dist2(dot3(x, y, z, x, y, z));
+  k.ir0 = div_normal(1.0, k.r0);
   const double v2 = dot3(vx, vy, vz, vx, vy, vz);
   k.eta0 = dot3(x, y, z, vx, vy, vz);
   k.beta = fma(2.0 * mu, k.ir0, -v2);
@@ -205,7 +239,7 @@ __device__ __forceinline__ KepSol kepler_solve(const KepStart &k, double s, doub
     const double F = fma(k.r0, s, fma(k.eta0, o.G2, fma(k.zeta0, o.G3, -dt)));
     const double rp = fma(k.eta0, o.G1, fma(k.zeta0, o.G2, k.r0));
     const double rpp = fma(k.eta0, o.G0, k.zeta0 * o.G1);
-    o.ds = F / rp;
+    o.ds = div_normal(F, rp);
     const double as = fabs(s);
     if (fabs(o.ds) <= 2e-6 * as && fabs(rpp) * (o.ds * o.ds) <= 4.4e-16 * (fabs(rp) * as)) {
       o.ok = true;
@@ -244,7 +278,7 @@ __device__ __forceinline__ int kepler_drift_from(const KepStart &k, double s, do
   const double G2n = o.G2 + fma(o.G1, hs, t0);
   const double G3n = o.G3 + fma(o.G2, hs, t1);
   const double r = fma(k.eta0, G1n, fma(k.zeta0, G2n, k.r0));
-  const double ir = 1.0 / r;
+  const double ir = div_normal(1.0, r);
   const double fm1 = -(mu * G2n) * k.ir0;
   const double g = fma(-mu, G3n, dt);
   const double fdot = -(mu * G1n) * (k.ir0 * ir);
@@ -277,7 +311,7 @@ __device__ __forceinline__ int kepler_drift_series(double &x, double &y, double 
 // The kick weight is K * m / ((r^2 + eps^2) r) = num / den.
 __device__ __forceinline__ void kick_nd(double r2, double eps2, double m, double rclo,
                                         double rcwinv, double &num, double &den) {
-  const double r = sqrt(r2);
+  const double r = sqrt_dist2(r2);
   const double D = (r2 + eps2) * r;
   const double yv = (r - rclo) * rcwinv;
   num = m;
@@ -300,7 +334,7 @@ __device__ __forceinline__ void batched_div(const double *num, const double *den
   P[0] = den[0];
 #pragma unroll
   for (int i = 1; i < N; ++i) P[i] = P[i - 1] * den[i];
-  double inv = 1.0 / P[N - 1];
+  double inv = div_normal(1.0, P[N - 1]);
 #pragma unroll
   for (int i = N - 1; i >= 1; --i) {
     w[i] = num[i] * (inv * P[i - 1]);
@@ -313,7 +347,7 @@ __device__ __forceinline__ void batched_div(const double *num, const double *den
 // zero for y >= 1 (replaces the double-counting weight of src/sim.cuh:357-361).
 __device__ __forceinline__ double close_weight(double r2, double eps2, double m, double rclo,
                                                double rcwinv) {
-  const double r = sqrt(r2);
+  const double r = sqrt_dist2(r2);
   const double yv = (r - rclo) * rcwinv;
   if (yv >= 1.0) return 0.0;
   const double D = (r2 + eps2) * r;
@@ -449,7 +483,7 @@ __device__ __forceinline__ void rhs_tp_at(const double *yv, double px, double py
   f[1] = yv[4];
   f[2] = yv[5];
   const double r2 = dot3(x, y, z, x, y, z);
-  const double r = sqrt(r2);
+  const double r = sqrt_dist2(r2);
   const double w0 = mu / (r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
   double w, dx, dy, dz;
@@ -477,7 +511,7 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
   f[1] = yv[4];
   f[2] = yv[5];
   const double r2 = dot3(x, y, z, x, y, z);
-  const double r = sqrt(r2);
+  const double r = sqrt_dist2(r2);
   const double w0 = mu / (r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
   const int ci = cache_point(tau, h);
@@ -742,7 +776,7 @@ __device__ __forceinline__ void rhs_group(const double *s, int g, const int *idx
     f[6 * a + 1] = s[6 * a + 4];
     f[6 * a + 2] = s[6 * a + 5];
     const double r2 = dot3(x, y, z, x, y, z);
-    const double r = sqrt(r2);
+    const double r = sqrt_dist2(r2);
     const double w0 = mu / (r2 * r);
     double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
     for (int b = 0; b < g; ++b) {

@@ -111,6 +111,7 @@ _SIGS = {
     "mint_reset_counters": (C.c_int, [C.c_void_p]),
     "mint_set_mode": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "mint_hill_radii": (C.c_int, [C.c_int64, C.c_double, C.c_double, _DP, _DP, _DP, _DP]),
+    "mint_selftest_arith": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, _DP, _DP, _DP]),
     "mint_elements_to_cartesian": (C.c_int, [C.c_int64, C.c_double] + [_DP] * 8),
     "mint_cartesian_to_elements": (C.c_int, [C.c_int64, C.c_double] + [_DP] * 8),
     "mint_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
@@ -402,6 +403,18 @@ class Mint:
 
 
 # host helpers (no GPU needed) -------------------------------------------------
+def selftest_arith(op, a, b=None, device=0):
+    """Device square root of squared distances (op 0) or normal-range
+    quotients a / b (op 1), for checks against the host's IEEE results."""
+    a = np.ascontiguousarray(a, np.float64)
+    out = np.empty_like(a)
+    bb = np.ascontiguousarray(b, np.float64) if b is not None else None
+    rc = lib().mint_selftest_arith(device, op, a.size, _d(a), _d(bb) if bb is not None else None, _d(out))
+    if rc != 0:
+        raise MintError(f"mint_selftest_arith failed ({rc})")
+    return out
+
+
 def elements_to_cartesian(mu, inc, node, argp, manom, ecc, sma):
     n = len(inc)
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (inc, node, argp, manom, ecc, sma)]
