@@ -357,7 +357,7 @@ __device__ __forceinline__ double close_weight(double r2, double eps2, double m,
     num = m * (om * om);
     den = D * fma(2.0, yv * yv, fma(-2.0, yv, 1.0));
   }
-  return num / den;
+  return div_normal(num, den);
 }
 
 // TP kick acceleration from the NM massive bodies at positions X (body-major
@@ -484,7 +484,7 @@ __device__ __forceinline__ void rhs_tp_at(const double *yv, double px, double py
   f[2] = yv[5];
   const double r2 = dot3(x, y, z, x, y, z);
   const double r = sqrt_dist2(r2);
-  const double w0 = mu / (r2 * r);
+  const double w0 = div_normal(mu, r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
   double w, dx, dy, dz;
   pull_at<NM>(jl < NM ? jl : NM - 1, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
@@ -512,7 +512,7 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
   f[2] = yv[5];
   const double r2 = dot3(x, y, z, x, y, z);
   const double r = sqrt_dist2(r2);
-  const double w0 = mu / (r2 * r);
+  const double w0 = div_normal(mu, r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
   const int ci = cache_point(tau, h);
   if (G == 1) {
@@ -627,8 +627,8 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
     double f0[6];
     rhs_tp<NM, G>(t, yv, Y, W, P4, h, prm, mu, eps2, f0, jl, base);
     nrhs += 1;
-    const double xs = sqrt(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
-    const double vs = sqrt(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
+    const double xs = sqrt_dist2(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
+    const double vs = sqrt_dist2(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
     double cur[6];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
@@ -640,7 +640,7 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
           double *slot = Rt + ((j - 1) * 6 + i) * rs;
           const double prev = *slot;
           *slot = cur[i];
-          cur[i] = cur[i] + (cur[i] - prev) / fac;
+          cur[i] = cur[i] + div_normal(cur[i] - prev, fac);
         }
       }
 #pragma unroll
@@ -649,7 +649,7 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
         double err = 0.0;
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-          const double e = fabs(cur[i] - Rt[((k - 1) * 6 + i) * rs]) / (i < 3 ? xs : vs);
+          const double e = div_normal(fabs(cur[i] - Rt[((k - 1) * 6 + i) * rs]), i < 3 ? xs : vs);
           err = e > err ? e : err;
         }
         if (err <= tol) {
@@ -777,7 +777,7 @@ __device__ __forceinline__ void rhs_group(const double *s, int g, const int *idx
     f[6 * a + 2] = s[6 * a + 5];
     const double r2 = dot3(x, y, z, x, y, z);
     const double r = sqrt_dist2(r2);
-    const double w0 = mu / (r2 * r);
+    const double w0 = div_normal(mu, r2 * r);
     double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
     for (int b = 0; b < g; ++b) {
       if (b == a) continue;
@@ -838,9 +838,9 @@ __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx
     rhs_group<NM>(s, g, idx, prm, mu, eps2, f0);
     ++nrhs;
     for (int a = 0; a < g; ++a) {
-      sc[2 * a] = sqrt(dot3(s[6 * a], s[6 * a + 1], s[6 * a + 2], s[6 * a], s[6 * a + 1], s[6 * a + 2]));
+      sc[2 * a] = sqrt_dist2(dot3(s[6 * a], s[6 * a + 1], s[6 * a + 2], s[6 * a], s[6 * a + 1], s[6 * a + 2]));
       sc[2 * a + 1] =
-          sqrt(dot3(s[6 * a + 3], s[6 * a + 4], s[6 * a + 5], s[6 * a + 3], s[6 * a + 4], s[6 * a + 5]));
+          sqrt_dist2(dot3(s[6 * a + 3], s[6 * a + 4], s[6 * a + 5], s[6 * a + 3], s[6 * a + 4], s[6 * a + 5]));
     }
     double R[KMAX][D];
     double cur[D];
@@ -853,7 +853,7 @@ __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx
         for (int i = 0; i < n; ++i) {
           const double prev = R[j - 1][i];
           R[j - 1][i] = cur[i];
-          cur[i] = cur[i] + (cur[i] - prev) / fac;
+          cur[i] = cur[i] + div_normal(cur[i] - prev, fac);
         }
       }
       for (int i = 0; i < n; ++i) R[k][i] = cur[i];
@@ -861,7 +861,7 @@ __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx
         double err = 0.0;
         for (int i = 0; i < n; ++i) {
           const int b = i / 6, c = i % 6;
-          const double e = fabs(cur[i] - R[k - 1][i]) / sc[2 * b + (c < 3 ? 0 : 1)];
+          const double e = div_normal(fabs(cur[i] - R[k - 1][i]), sc[2 * b + (c < 3 ? 0 : 1)]);
           err = e > err ? e : err;
         }
         if (err <= tol) {

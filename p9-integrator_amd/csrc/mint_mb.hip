@@ -498,8 +498,8 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
     f[1] = z[4];
     f[2] = z[5];
     const double r2 = dot3(x, yy, zz, x, yy, zz);
-    const double r = sqrt(r2);
-    const double w0 = mu / (r2 * r);
+    const double r = sqrt_dist2(r2);
+    const double w0 = div_normal(mu, r2 * r);
     double ax = -(w0 * x), ay = -(w0 * yy), az = -(w0 * zz);
     for (int b = 0; b < gs; ++b) {
       const double xb = __shfl(x, b, 64), yb = __shfl(yy, b, 64), zb = __shfl(zz, b, 64);
@@ -524,8 +524,8 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
     double f0[6];
     rhs(y, f0);
     ++nrhs;
-    const double scx = sqrt(dot3(y[0], y[1], y[2], y[0], y[1], y[2]));
-    const double scv = sqrt(dot3(y[3], y[4], y[5], y[3], y[4], y[5]));
+    const double scx = sqrt_dist2(dot3(y[0], y[1], y[2], y[0], y[1], y[2]));
+    const double scv = sqrt_dist2(dot3(y[3], y[4], y[5], y[3], y[4], y[5]));
     double R[KMAX][6], cur[6];
     bool conv = false;
 #pragma unroll
@@ -558,7 +558,7 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
         for (int c = 0; c < 6; ++c) {
           const double prev = R[j - 1][c];
           R[j - 1][c] = cur[c];
-          cur[c] = cur[c] + (cur[c] - prev) / fac;
+          cur[c] = cur[c] + div_normal(cur[c] - prev, fac);
         }
       }
 #pragma unroll
@@ -568,7 +568,7 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
         if (act) {
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
-            const double e = fabs(cur[c] - R[k - 1][c]) / (c < 3 ? scx : scv);
+            const double e = div_normal(fabs(cur[c] - R[k - 1][c]), c < 3 ? scx : scv);
             err = e > err ? e : err;
           }
         }
