@@ -395,16 +395,20 @@ template <int NM>
 __device__ __forceinline__ bool encounter_tp(double x, double y, double z,
                                              const double *__restrict__ Y,
                                              const double *__restrict__ prm) {
-  bool f = false;
+  // every pair, no short-circuit branches: r2 < rc2 exactly when r2 - rc2 < 0
+  // (the difference of two unequal finite doubles is never rounded to zero),
+  // so the smallest difference decides; a NaN pair never counts, as in the
+  // comparison (fmin drops it)
+  double m = INFINITY;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     const double dx = Y[3 * j + 0] - x;
     const double dy = Y[3 * j + 1] - y;
     const double dz = Y[3 * j + 2] - z;
     const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-    f |= (r2 < prm[P_RC2(NM) + j]);  // every pair, no short-circuit branches
+    m = fmin(m, r2 - prm[P_RC2(NM) + j]);
   }
-  return f;
+  return m < 0.0;
 }
 
 // Right-hand side of the close-encounter problem of one TP at time tau into
