@@ -471,14 +471,16 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 
   double X[P][3], V[P][3], A[P][3];
   bool act[P];
-  long long idx[P];
-  unsigned int n_kep = 0, n_newton = 0, n_defer = 0, n_idle = 0, n_step = 0;
+  int idx[P];  // TPs per context < 2^31
+  // work counters: every step the main kernel completes is one Kepler solve,
+  // and the lane-steps it leaves idle are C per TP slot minus those steps
+  unsigned int n_newton = 0, n_defer = 0, n_step = 0;
   uint32_t st = 0;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int loc = p * (int)blockDim.x + (int)threadIdx.x;
     act[p] = loc < tile.count;
-    idx[p] = tile.start + loc;
+    idx[p] = (int)(tile.start + loc);
     if (act[p]) {
       X[p][0] = tp.x[idx[p]]; X[p][1] = tp.y[idx[p]]; X[p][2] = tp.z[idx[p]];
       V[p][0] = tp.vx[idx[p]]; V[p][1] = tp.vy[idx[p]]; V[p][2] = tp.vz[idx[p]];
@@ -489,10 +491,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     // are the encounter kernel's for the whole window
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      if (act[p] && pre[idx[p]]) {
-        act[p] = false;
-        n_idle += (unsigned)C;
-      }
+      if (act[p] && pre[idx[p]]) act[p] = false;
   }
 #pragma unroll
   for (int p = 0; p < P; ++p)
@@ -524,11 +523,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
         if (carry != nullptr) carry[idx[p]] = 1;  // queued for the next window too
         act[p] = false;
         ++n_defer;
-        n_idle += (unsigned)(C - k);
         continue;
       }
       n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
-      ++n_kep;
       x = fma(sc.hh, r[R_Q2(NM) + 0], x);
       y = fma(sc.hh, r[R_Q2(NM) + 1], y);
       z = fma(sc.hh, r[R_Q2(NM) + 2], z);
@@ -551,11 +548,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
   }
   if (st) atomicOr(&status[s], st);
   if (stats) {
+    unsigned int slots = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) slots += (p * (int)blockDim.x + (int)threadIdx.x < tile.count) ? 1u : 0u;
     wave_add(stats + S_TPSTEPS, n_step);
-    wave_add(stats + S_KEPLER, n_kep);
+    wave_add(stats + S_KEPLER, n_step);
     wave_add(stats + S_NEWTON, n_newton);
     wave_add(stats + S_DEFER, n_defer);
-    wave_add(stats + S_IDLE, n_idle);
+    wave_add(stats + S_IDLE, slots * (unsigned)C - n_step);
   }
 }
 
