@@ -38,7 +38,7 @@ constexpr int MB_DIVG = 8;
 //   [4NM+NM^2, 4NM+2NM^2)    pair 1/(0.9 r_c)
 //   [4NM+2NM^2, 4NM+3NM^2)   pair r_c^2
 //   [4NM+3NM^2, 5NM+3NM^2)   physical radius R_j (0 = none)
-__host__ __device__ constexpr int prm_size(int nm) { return ((5 * nm + 3 * nm * nm) + 1) & ~1; }
+__host__ __device__ constexpr int prm_size(int nm) { return ((6 * nm + 3 * nm * nm) + 1) & ~1; }
 __host__ __device__ constexpr int P_M(int) { return 0; }
 __host__ __device__ constexpr int P_RCLO(int nm) { return nm; }
 __host__ __device__ constexpr int P_RCWINV(int nm) { return 2 * nm; }
@@ -47,6 +47,10 @@ __host__ __device__ constexpr int P_PRCLO(int nm) { return 4 * nm; }
 __host__ __device__ constexpr int P_PRCWINV(int nm) { return 4 * nm + nm * nm; }
 __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; }
 __host__ __device__ constexpr int P_RAD(int nm) { return 4 * nm + 3 * nm * nm; }
+// r_c^2 (1 + 1e-9): a TP farther than this from body j has K = 1 whatever the
+// rounding of y = (r - 0.1 r_c) / (0.9 r_c) (its computed y exceeds 1 by
+// ~5e-10), so the kick skips forming y there
+__host__ __device__ constexpr int P_FAR2(int nm) { return 5 * nm + 3 * nm * nm; }
 
 // Perturber cache: for every system and step, the massive positions
 // Kepler-drifted from (Y, W) by tau = m (h/16), m = 1..16, the substep times of
@@ -374,8 +378,13 @@ __device__ __forceinline__ void accel_tp(double x, double y, double z,
     dy[j] = X[3 * j + 1] - y;
     dz[j] = X[3 * j + 2] - z;
     const double r2 = dot3(dx[j], dy[j], dz[j], dx[j], dy[j], dz[j]);
-    kick_nd(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
-            den[j]);
+    if (r2 > prm[P_FAR2(NM) + j]) {  // K = 1: the weight kick_nd forms there
+      num[j] = prm[P_M(NM) + j];
+      den[j] = (r2 + eps2) * sqrt_dist2(r2);
+    } else {
+      kick_nd(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
+              den[j]);
+    }
   }
   batched_div<NM>(num, den, w);
   ax = 0.0;
