@@ -428,7 +428,7 @@ int mint_upload(void *ctx, const mint_state *st) {
       p[P_RCLO(NM) + j] = 0.1 * rcs[j];
       p[P_RCWINV(NM) + j] = 1.0 / (0.9 * rcs[j]);
       p[P_RC2(NM) + j] = rcs[j] * rcs[j];
-      p[P_FAR2(NM) + j] = (rcs[j] * rcs[j]) * (1.0 + 1e-9);
+      p[P_FAR2(NM) + j] = (rcs[j] * rcs[j]) * (1.0 + 1e-3);
       for (int k = 0; k < NM; ++k) {
         const double r = std::max(rcs[j], rcs[k]);
         p[P_PRCLO(NM) + j * NM + k] = 0.1 * r;

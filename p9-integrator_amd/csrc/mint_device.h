@@ -47,9 +47,13 @@ __host__ __device__ constexpr int P_PRCLO(int nm) { return 4 * nm; }
 __host__ __device__ constexpr int P_PRCWINV(int nm) { return 4 * nm + nm * nm; }
 __host__ __device__ constexpr int P_PRC2(int nm) { return 4 * nm + 2 * nm * nm; }
 __host__ __device__ constexpr int P_RAD(int nm) { return 4 * nm + 3 * nm * nm; }
-// r_c^2 (1 + 1e-9): a TP farther than this from body j has K = 1 whatever the
-// rounding of y = (r - 0.1 r_c) / (0.9 r_c) (its computed y exceeds 1 by
-// ~5e-10), so the kick skips forming y there
+// Far radius r_c^2 (1 + 1e-3) of body j.  A TP whose kick finds it farther
+// than this (i) has K = 1 whatever the rounding of y = (r - 0.1 r_c) /
+// (0.9 r_c) (the computed y exceeds 1 by ~5e-4), so the kick skips forming
+// y; and (ii) is outside r_c at the next step's encounter check: that check
+// measures the same separation after both ends moved by the same jump, which
+// changes it by a few ulps of the positions (far below 5e-4 r_c for bodies
+// within 1e8 AU and r_c above 2e-4 AU).
 __host__ __device__ constexpr int P_FAR2(int nm) { return 5 * nm + 3 * nm * nm; }
 
 // Perturber cache: for every system and step, the massive positions
@@ -366,12 +370,14 @@ __device__ __forceinline__ double close_weight(double r2, double eps2, double m,
 
 // TP kick acceleration from the NM massive bodies at positions X (body-major
 // xyz).  Replaces body_interaction_kick for a massless body (src/sim.cuh:327-382).
+// far: every pair is beyond its far radius (see P_FAR2).
 template <int NM>
 __device__ __forceinline__ void accel_tp(double x, double y, double z,
                                          const double *__restrict__ X,
                                          const double *__restrict__ prm, double eps2,
-                                         double &ax, double &ay, double &az) {
+                                         double &ax, double &ay, double &az, bool &far) {
   double dx[NM], dy[NM], dz[NM], num[NM], den[NM], w[NM];
+  far = true;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     dx[j] = X[3 * j + 0] - x;
@@ -382,6 +388,7 @@ __device__ __forceinline__ void accel_tp(double x, double y, double z,
       num[j] = prm[P_M(NM) + j];
       den[j] = (r2 + eps2) * sqrt_dist2(r2);
     } else {
+      far = false;
       kick_nd(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
               den[j]);
     }
@@ -396,6 +403,15 @@ __device__ __forceinline__ void accel_tp(double x, double y, double z,
     ay = fma(w[j], dy[j], ay);
     az = fma(w[j], dz[j], az);
   }
+}
+
+template <int NM>
+__device__ __forceinline__ void accel_tp(double x, double y, double z,
+                                         const double *__restrict__ X,
+                                         const double *__restrict__ prm, double eps2,
+                                         double &ax, double &ay, double &az) {
+  bool far;
+  accel_tp<NM>(x, y, z, X, prm, eps2, ax, ay, az, far);
 }
 
 // The exact distance check: is the TP within r_c,j of any massive body?

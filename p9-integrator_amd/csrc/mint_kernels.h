@@ -493,9 +493,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     for (int p = 0; p < P; ++p)
       if (act[p] && pre[idx[p]]) act[p] = false;
   }
+  // far[p]: the last kick found every pair beyond its far radius, so the next
+  // encounter check cannot fire (P_FAR2) and is skipped
+  bool far[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p)
-    if (act[p]) accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2]);
+  for (int p = 0; p < P; ++p) {
+    far[p] = false;
+    if (act[p])
+      accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2], far[p]);
+  }
 
   for (int k = 0; k < C; ++k) {
     // The step's whole ephemeris record is loaded up front (uniform, so one
@@ -516,7 +522,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
       x = fma(sc.hh, r[R_Q1(NM) + 0], x);
       y = fma(sc.hh, r[R_Q1(NM) + 1], y);
       z = fma(sc.hh, r[R_Q1(NM) + 2], z);
-      if (encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
+      if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
         const unsigned int slot = atomicAdd(qcount, 1u);
         q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
         flag_system(need, 2u, s, slist, qcount + 3);
@@ -530,7 +536,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
       y = fma(sc.hh, r[R_Q2(NM) + 1], y);
       z = fma(sc.hh, r[R_Q2(NM) + 2], z);
       double ax, ay, az;
-      accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az);
+      accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az, far[p]);
       A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
       X[p][0] = x; X[p][1] = y; X[p][2] = z;
       V[p][0] = fma(sc.hh, ax, vx);
