@@ -93,7 +93,8 @@ struct StepConst {
 // reciprocal-square-root refinement the compiler uses for sqrt, without its
 // rescaling of inputs below 2^-767 (which a squared distance in AU never is).
 // Zero and infinity are returned as they are.
-__device__ __forceinline__ double sqrt_dist2(double x) {
+// sqrt_dist2 of a positive finite x: the refinement alone.
+__device__ __forceinline__ double sqrt_dist2_pos(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = y * 0.5;
   const double r = fma(-h, g, 0.5);
@@ -102,7 +103,11 @@ __device__ __forceinline__ double sqrt_dist2(double x) {
   double d = fma(-g, g, x);
   g = fma(d, h, g);
   d = fma(-g, g, x);
-  g = fma(d, h, g);
+  return fma(d, h, g);
+}
+
+__device__ __forceinline__ double sqrt_dist2(double x) {
+  const double g = sqrt_dist2_pos(x);
   return (x == 0.0 || x == INFINITY) ? x : g;
 }
 
@@ -317,9 +322,9 @@ __device__ __forceinline__ int kepler_drift_series(double &x, double &y, double 
 // Changeover-weighted pair acceleration factor (src/sim.cuh:166-186,327-382):
 // y = (r - 0.1 r_c)/(0.9 r_c); K = 0 (y<=0), y^2/(2y^2-2y+1) (0<y<1), 1 (y>=1).
 // The kick weight is K * m / ((r^2 + eps^2) r) = num / den.
-__device__ __forceinline__ void kick_nd(double r2, double eps2, double m, double rclo,
-                                        double rcwinv, double &num, double &den) {
-  const double r = sqrt_dist2(r2);
+// r = |r2|^(1/2) is passed in.
+__device__ __forceinline__ void kick_nd_r(double r2, double r, double eps2, double m, double rclo,
+                                          double rcwinv, double &num, double &den) {
   const double D = (r2 + eps2) * r;
   const double yv = (r - rclo) * rcwinv;
   num = m;
@@ -331,6 +336,20 @@ __device__ __forceinline__ void kick_nd(double r2, double eps2, double m, double
     num = m * y2;
     den = D * fma(2.0, y2, fma(-2.0, yv, 1.0));
   }
+}
+
+__device__ __forceinline__ void kick_nd(double r2, double eps2, double m, double rclo,
+                                        double rcwinv, double &num, double &den) {
+  kick_nd_r(r2, sqrt_dist2(r2), eps2, m, rclo, rcwinv, num, den);
+}
+
+// kick_nd with the square root of a positive finite r2.  Where the weights go
+// on to one batched division (accel_tp), r2 = 0 or infinity gives NaN weights
+// with either root (the reciprocal of a zero or infinite product is NaN), so
+// the zero and infinity cases of sqrt_dist2 change nothing there.
+__device__ __forceinline__ void kick_nd_pos(double r2, double eps2, double m, double rclo,
+                                            double rcwinv, double &num, double &den) {
+  kick_nd_r(r2, sqrt_dist2_pos(r2), eps2, m, rclo, rcwinv, num, den);
 }
 
 // w_i = num_i / den_i for n terms with one division (batched reciprocal):
@@ -386,11 +405,11 @@ __device__ __forceinline__ void accel_tp(double x, double y, double z,
     const double r2 = dot3(dx[j], dy[j], dz[j], dx[j], dy[j], dz[j]);
     if (r2 > prm[P_FAR2(NM) + j]) {  // K = 1: the weight kick_nd forms there
       num[j] = prm[P_M(NM) + j];
-      den[j] = (r2 + eps2) * sqrt_dist2(r2);
+      den[j] = (r2 + eps2) * sqrt_dist2_pos(r2);
     } else {
       far = false;
-      kick_nd(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
-              den[j]);
+      kick_nd_pos(r2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j], num[j],
+                  den[j]);
     }
   }
   batched_div<NM>(num, den, w);

@@ -504,10 +504,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
   }
 
   for (int k = 0; k < C; ++k) {
-    // The step's whole ephemeris record is loaded up front (uniform, so one
-    // round of scalar loads whose latency overlaps the kick); loaded where
-    // each value is used, the loads would sit behind the divergent branches
-    // and each wait would be exposed.
+    // The step's jump momenta and check positions are loaded up front
+    // (uniform, so one round of scalar loads whose latency overlaps the kick;
+    // loaded where each value is used, the loads would sit behind the
+    // divergent branches and each wait would be exposed).  The kick positions
+    // are loaded after the Kepler drift: held across it, they overflow the
+    // scalar registers into vector-lane spills (1.3 % slower).
     const double *rg = rsys + (size_t)__builtin_amdgcn_readfirstlane(k) * REC;
     double r[REC];
 #pragma unroll
@@ -532,11 +534,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
         continue;
       }
       n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
+      double xe[3 * NM];
+#pragma unroll
+      for (int i = 0; i < 3 * NM; ++i) xe[i] = rg[R_XE(NM) + i];
       x = fma(sc.hh, r[R_Q2(NM) + 0], x);
       y = fma(sc.hh, r[R_Q2(NM) + 1], y);
       z = fma(sc.hh, r[R_Q2(NM) + 2], z);
       double ax, ay, az;
-      accel_tp<NM>(x, y, z, r + R_XE(NM), prm, sc.eps2, ax, ay, az, far[p]);
+      accel_tp<NM>(x, y, z, xe, prm, sc.eps2, ax, ay, az, far[p]);
       A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
       X[p][0] = x; X[p][1] = y; X[p][2] = z;
       V[p][0] = fma(sc.hh, ax, vx);
