@@ -200,9 +200,11 @@ int mint_reset_counters(void *ctx);
 int mint_set_mode(void *ctx, int32_t form, int32_t kepler_guess, int32_t use_scorer);
 
 /* Diagnostic (no counterpart in the reference): evaluate the device's
- * square root of a squared distance (op 0: out = sqrt(a)) or normal-range
- * quotient (op 1: out = a / b) on n inputs on device `device`, so callers can
- * check them against the host's IEEE results.  b may be NULL for op 0. */
+ * square root of a squared distance (op 0: out = sqrt(a)), its normal-range
+ * quotient (op 1: out = a / b) or the square root the test-particle kick uses
+ * for positive finite squared distances (op 2: out = sqrt(a), without the
+ * zero and infinity cases) on n inputs on device `device`, so callers can
+ * check them against the host's IEEE results.  b may be NULL for ops 0, 2. */
 int mint_selftest_arith(int32_t device, int32_t op, int64_t n, const double *a, const double *b,
                         double *out);
 

@@ -238,7 +238,13 @@ namespace {
 __global__ void selftest_arith_kernel(int op, long long n, const double *__restrict__ a,
                                       const double *__restrict__ b, double *__restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = op == 0 ? mint::sqrt_dist2(a[i]) : mint::div_normal(a[i], b[i]);
+  if (i >= n) return;
+  if (op == 0)
+    out[i] = mint::sqrt_dist2(a[i]);
+  else if (op == 1)
+    out[i] = mint::div_normal(a[i], b[i]);
+  else
+    out[i] = mint::sqrt_dist2_pos(a[i]);
 }
 }  // namespace
 
@@ -334,7 +340,7 @@ const char *mint_last_error(void *ctx) {
 
 int mint_selftest_arith(int32_t device, int32_t op, int64_t n, const double *a, const double *b,
                         double *out) {
-  if (n < 0 || !a || !out || (op != 0 && op != 1) || (op == 1 && !b)) return MINT_EINVAL;
+  if (n < 0 || !a || !out || op < 0 || op > 2 || (op == 1 && !b)) return MINT_EINVAL;
   if (n == 0) return MINT_OK;
   if (hipSetDevice(device) != hipSuccess) return MINT_EHIP;
   const size_t bytes = (size_t)n * sizeof(double);

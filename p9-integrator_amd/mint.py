@@ -404,8 +404,9 @@ class Mint:
 
 # host helpers (no GPU needed) -------------------------------------------------
 def selftest_arith(op, a, b=None, device=0):
-    """Device square root of squared distances (op 0) or normal-range
-    quotients a / b (op 1), for checks against the host's IEEE results."""
+    """Device square root of squared distances (op 0), normal-range
+    quotients a / b (op 1) or the kick's square root of positive finite
+    squared distances (op 2), for checks against the host's IEEE results."""
     a = np.ascontiguousarray(a, np.float64)
     out = np.empty_like(a)
     bb = np.ascontiguousarray(b, np.float64) if b is not None else None

@@ -47,3 +47,24 @@ def test_normal_range_quotient_is_ieee():
     ref = A / B
     bad = np.flatnonzero(out.view(np.uint64) != ref.view(np.uint64))
     assert bad.size == 0, (bad.size, A[bad[:5]], B[bad[:5]], out[bad[:5]], ref[bad[:5]])
+
+
+def test_kick_square_root_and_its_special_cases():
+    """The kick's square root (no zero / infinity cases) is IEEE on every
+    positive finite squared distance; at zero and infinity it gives NaN, and
+    so does the reciprocal of a zero or infinite product of denominators, so
+    the kick's NaN weights there do not depend on which root was taken."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([
+        _loguniform(rng, -700.0, 700.0, 500_000),
+        _loguniform(rng, -40.0, 40.0, 500_000),
+        np.square(rng.integers(1, 2**26, 100_000).astype(np.float64)),
+    ])
+    out = mint.selftest_arith(2, x)
+    ref = np.sqrt(x)
+    bad = np.flatnonzero(out.view(np.uint64) != ref.view(np.uint64))
+    assert bad.size == 0, (bad.size, x[bad[:5]], out[bad[:5]], ref[bad[:5]])
+    special = np.array([0.0, np.inf])
+    assert np.isnan(mint.selftest_arith(2, special)).all()
+    assert (mint.selftest_arith(0, special) == special).all()
+    assert np.isnan(mint.selftest_arith(1, np.ones(3), np.array([0.0, np.inf, np.nan]))).all()
