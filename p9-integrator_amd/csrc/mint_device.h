@@ -145,31 +145,33 @@ __device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double
   }
   // Series length by |z| band: the first omitted term is below 3e-19 of the
   // sum in every band (|z| <= 1e-5: degree 2, <= 1e-3: degree 3, else 6).
+  // The degree-2 band (every drift of a distant body) is formed first and
+  // without a branch; the others replace it.
   const double az = fabs(z);
-  double t2, t3;
-  if (az <= 1e-5) {
-    t2 = 0.001388888888888889;
-    t3 = 0.0001984126984126984;
-  } else if (az <= 1e-3) {
-    t2 = -2.48015873015873e-05;
-    t3 = -2.7557319223985893e-06;
-    t2 = fma(t2, z, 0.001388888888888889);
-    t3 = fma(t3, z, 0.0001984126984126984);
-  } else {
-    t2 = 1.1470745597729725e-11;
-    t2 = fma(t2, z, -2.08767569878681e-09);
-    t2 = fma(t2, z, 2.755731922398589e-07);
-    t2 = fma(t2, z, -2.48015873015873e-05);
-    t2 = fma(t2, z, 0.001388888888888889);
-    t3 = 7.647163731819816e-13;
-    t3 = fma(t3, z, -1.6059043836821613e-10);
-    t3 = fma(t3, z, 2.505210838544172e-08);
-    t3 = fma(t3, z, -2.7557319223985893e-06);
-    t3 = fma(t3, z, 0.0001984126984126984);
+  double t2 = fma(0.001388888888888889, z, -0.041666666666666664);
+  double t3 = fma(0.0001984126984126984, z, -0.008333333333333333);
+  if (!(az <= 1e-5)) {
+    if (az <= 1e-3) {
+      t2 = -2.48015873015873e-05;
+      t3 = -2.7557319223985893e-06;
+      t2 = fma(t2, z, 0.001388888888888889);
+      t3 = fma(t3, z, 0.0001984126984126984);
+    } else {
+      t2 = 1.1470745597729725e-11;
+      t2 = fma(t2, z, -2.08767569878681e-09);
+      t2 = fma(t2, z, 2.755731922398589e-07);
+      t2 = fma(t2, z, -2.48015873015873e-05);
+      t2 = fma(t2, z, 0.001388888888888889);
+      t3 = 7.647163731819816e-13;
+      t3 = fma(t3, z, -1.6059043836821613e-10);
+      t3 = fma(t3, z, 2.505210838544172e-08);
+      t3 = fma(t3, z, -2.7557319223985893e-06);
+      t3 = fma(t3, z, 0.0001984126984126984);
+    }
+    t2 = fma(t2, z, -0.041666666666666664);
+    t3 = fma(t3, z, -0.008333333333333333);
   }
-  t2 = fma(t2, z, -0.041666666666666664);
   t2 = fma(t2, z, 0.5);
-  t3 = fma(t3, z, -0.008333333333333333);
   t3 = fma(t3, z, 0.16666666666666666);
   c2 = t2;
   c3 = t3;
@@ -233,46 +235,58 @@ struct KepSol {
   bool ok;
 };
 
+// One Newton evaluation at s: G-functions, F, F', F'' and the correction ds;
+// true when the stopping rule holds.
+__device__ __forceinline__ bool kepler_eval(const KepStart &k, double s, double dt, KepSol &o,
+                                            double &F, double &rp) {
+  double c0, c1, c2, c3;
+  const double ss = s * s;
+  stumpff(k.beta * ss, c0, c1, c2, c3);
+  o.G0 = c0;
+  o.G1 = s * c1;
+  o.G2 = ss * c2;
+  o.G3 = ss * (s * c3);
+  F = fma(k.r0, s, fma(k.eta0, o.G2, fma(k.zeta0, o.G3, -dt)));
+  rp = fma(k.eta0, o.G1, fma(k.zeta0, o.G2, k.r0));
+  const double rpp = fma(k.eta0, o.G0, k.zeta0 * o.G1);
+  o.ds = div_normal(F, rp);
+  const double as = fabs(s);
+  return fabs(o.ds) <= 2e-6 * as && fabs(rpp) * (o.ds * o.ds) <= 4.4e-16 * (fabs(rp) * as);
+}
+
+// The first iteration is written out (it is the last one for almost every
+// drift), the rest loop.
 __device__ __forceinline__ KepSol kepler_solve(const KepStart &k, double s, double dt) {
   KepSol o;
-  o.it = 0;
-  o.ok = false;
-  // F is increasing (F' = r > 0) and F(0) = -dt, so the root lies in (lo, hi).
-  double lo = dt >= 0.0 ? 0.0 : -INFINITY;
-  double hi = dt >= 0.0 ? INFINITY : 0.0;
-  while (true) {
-    ++o.it;
-    double c0, c1, c2, c3;
-    const double ss = s * s;
-    stumpff(k.beta * ss, c0, c1, c2, c3);
-    o.G0 = c0;
-    o.G1 = s * c1;
-    o.G2 = ss * c2;
-    o.G3 = ss * (s * c3);
-    const double F = fma(k.r0, s, fma(k.eta0, o.G2, fma(k.zeta0, o.G3, -dt)));
-    const double rp = fma(k.eta0, o.G1, fma(k.zeta0, o.G2, k.r0));
-    const double rpp = fma(k.eta0, o.G0, k.zeta0 * o.G1);
-    o.ds = div_normal(F, rp);
-    const double as = fabs(s);
-    if (fabs(o.ds) <= 2e-6 * as && fabs(rpp) * (o.ds * o.ds) <= 4.4e-16 * (fabs(rp) * as)) {
-      o.ok = true;
-      break;
+  double F, rp;
+  o.it = 1;
+  o.ok = kepler_eval(k, s, dt, o, F, rp);
+  if (!o.ok) {
+    // F is increasing (F' = r > 0) and F(0) = -dt, so the root lies in (lo, hi).
+    double lo = dt >= 0.0 ? 0.0 : -INFINITY;
+    double hi = dt >= 0.0 ? INFINITY : 0.0;
+    while (o.it < NEWTON_MAXIT) {
+      if (F < 0.0)
+        lo = s;
+      else
+        hi = s;
+      double sn = s - o.ds;
+      if (!(sn > lo && sn < hi)) {
+        // Newton left the bracket: Laguerre-Conway step (n = 5, Conway 1986),
+        // then bisection if that also leaves it.
+        const double rpp = fma(k.eta0, o.G0, k.zeta0 * o.G1);
+        const double d2 = fabs(fma(16.0 * rp, rp, -(20.0 * F * rpp)));
+        const double den = rp + copysign(sqrt(d2), rp);
+        sn = s - (5.0 * F) / den;
+        if (!(sn > lo && sn < hi)) sn = 0.5 * (lo + hi);
+      }
+      s = sn;
+      ++o.it;
+      if (kepler_eval(k, s, dt, o, F, rp)) {
+        o.ok = true;
+        break;
+      }
     }
-    if (o.it >= NEWTON_MAXIT) break;
-    if (F < 0.0)
-      lo = s;
-    else
-      hi = s;
-    double sn = s - o.ds;
-    if (!(sn > lo && sn < hi)) {
-      // Newton left the bracket: Laguerre-Conway step (n = 5, Conway 1986),
-      // then bisection if that also leaves it.
-      const double d2 = fabs(fma(16.0 * rp, rp, -(20.0 * F * rpp)));
-      const double den = rp + copysign(sqrt(d2), rp);
-      sn = s - (5.0 * F) / den;
-      if (!(sn > lo && sn < hi)) sn = 0.5 * (lo + hi);
-    }
-    s = sn;
   }
   o.s = s;
   return o;

@@ -503,6 +503,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
       accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2], far[p]);
   }
 
+  // The half step in a vector register: beside a jump momentum (a scalar
+  // register) it would take a second scalar operand, which one VOP3
+  // instruction cannot encode, so each jump would first copy one of them.
+  double hh = sc.hh;
+  asm volatile("" : "+v"(hh));
   for (int k = 0; k < C; ++k) {
     // The step's jump momenta and check positions are loaded up front
     // (uniform, so one round of scalar loads whose latency overlaps the kick;
@@ -521,9 +526,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
       double vx = fma(sc.hh, A[p][0], V[p][0]);
       double vy = fma(sc.hh, A[p][1], V[p][1]);
       double vz = fma(sc.hh, A[p][2], V[p][2]);
-      x = fma(sc.hh, r[R_Q1(NM) + 0], x);
-      y = fma(sc.hh, r[R_Q1(NM) + 1], y);
-      z = fma(sc.hh, r[R_Q1(NM) + 2], z);
+      x = fma(hh, r[R_Q1(NM) + 0], x);
+      y = fma(hh, r[R_Q1(NM) + 1], y);
+      z = fma(hh, r[R_Q1(NM) + 2], z);
       if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
         const unsigned int slot = atomicAdd(qcount, 1u);
         q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
@@ -537,9 +542,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
       double xe[3 * NM];
 #pragma unroll
       for (int i = 0; i < 3 * NM; ++i) xe[i] = rg[R_XE(NM) + i];
-      x = fma(sc.hh, r[R_Q2(NM) + 0], x);
-      y = fma(sc.hh, r[R_Q2(NM) + 1], y);
-      z = fma(sc.hh, r[R_Q2(NM) + 2], z);
+      x = fma(hh, r[R_Q2(NM) + 0], x);
+      y = fma(hh, r[R_Q2(NM) + 1], y);
+      z = fma(hh, r[R_Q2(NM) + 2], z);
       double ax, ay, az;
       accel_tp<NM>(x, y, z, xe, prm, sc.eps2, ax, ay, az, far[p]);
       A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
