@@ -81,6 +81,36 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst, unsigned int v
   if ((threadIdx.x & 63) == 0 && v != 0u) atomicAdd(dst, (unsigned long long)v);
 }
 
+// Wavefront-wide helpers for converged code (every lane of the wavefront
+// active).  wave_count: lanes where pred holds.  wave_sum: sum of v over the
+// wavefront.  wave_sum_rare: the same, read lane by lane from those with
+// v != 0 (cheap when they are few).  wave_add_uniform: add a wavefront-uniform
+// v to a global counter once.
+__device__ __forceinline__ unsigned int wave_count(bool pred) {
+  return (unsigned int)__builtin_popcountll(__builtin_amdgcn_ballot_w64(pred));
+}
+
+__device__ __forceinline__ unsigned int wave_sum(unsigned int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ unsigned int wave_sum_rare(unsigned int v) {
+  unsigned long long m = __builtin_amdgcn_ballot_w64(v != 0u);
+  unsigned int t = 0;
+  while (m != 0ull) {
+    const int l = __builtin_ctzll(m);
+    t += (unsigned int)__builtin_amdgcn_readlane((int)v, l);
+    m &= m - 1ull;
+  }
+  return t;
+}
+
+__device__ __forceinline__ void wave_add_uniform(unsigned long long *dst, unsigned int v) {
+  if ((threadIdx.x & 63) == 0 && v != 0u) atomicAdd(dst, (unsigned long long)v);
+}
+
 __device__ __forceinline__ bool finite6(double a, double b, double c, double d, double e,
                                         double f) {
   return isfinite(a) && isfinite(b) && isfinite(c) && isfinite(d) && isfinite(e) && isfinite(f);
@@ -472,10 +502,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
   double X[P][3], V[P][3], A[P][3];
   bool act[P];
   int idx[P];  // TPs per context < 2^31
-  // work counters: every step the main kernel completes is one Kepler solve,
-  // and the lane-steps it leaves idle are C per TP slot minus those steps
-  unsigned int n_newton = 0, n_defer = 0, n_step = 0;
-  uint32_t st = 0;
+  // Work counters and status bits are kept per wavefront in scalar registers
+  // and updated where the lanes have reconverged after each step, from
+  // ballots: nothing per lane is carried through the step loop, so nothing
+  // has to be saved around the learned guess's call.  Every step the kernel
+  // completes is one Kepler solve; the lane-steps it leaves idle are C per
+  // TP slot minus those steps; a TP leaves the kernel early only by deferral.
+  unsigned int n_step = 0, n_newton_extra = 0, st_wave = 0;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int loc = p * (int)blockDim.x + (int)threadIdx.x;
@@ -493,11 +526,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     for (int p = 0; p < P; ++p)
       if (act[p] && pre[idx[p]]) act[p] = false;
   }
+  unsigned int n_start = 0;
   // far[p]: the last kick found every pair beyond its far radius, so the next
   // encounter check cannot fire (P_FAR2) and is skipped
   bool far[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
+    n_start += wave_count(act[p]);
     far[p] = false;
     if (act[p])
       accel_tp<NM>(X[p][0], X[p][1], X[p][2], xsys, prm, sc.eps2, A[p][0], A[p][1], A[p][2], far[p]);
@@ -521,57 +556,65 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     for (int i = 0; i < REC; ++i) r[i] = rg[i];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      if (!act[p]) continue;
-      double x = X[p][0], y = X[p][1], z = X[p][2];
-      double vx = fma(sc.hh, A[p][0], V[p][0]);
-      double vy = fma(sc.hh, A[p][1], V[p][1]);
-      double vz = fma(sc.hh, A[p][2], V[p][2]);
-      x = fma(hh, r[R_Q1(NM) + 0], x);
-      y = fma(hh, r[R_Q1(NM) + 1], y);
-      z = fma(hh, r[R_Q1(NM) + 2], z);
-      if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
-        const unsigned int slot = atomicAdd(qcount, 1u);
-        q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
-        flag_system(need, 2u, s, slist, qcount + 3);
-        if (carry != nullptr) carry[idx[p]] = 1;  // queued for the next window too
-        act[p] = false;
-        ++n_defer;
-        continue;
-      }
-      n_newton += (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, st);
-      double xe[3 * NM];
+      unsigned int extra = 0;  // Newton iterations beyond the first
+      uint32_t stb = 0;
+      if (act[p]) {
+        double x = X[p][0], y = X[p][1], z = X[p][2];
+        double vx = fma(sc.hh, A[p][0], V[p][0]);
+        double vy = fma(sc.hh, A[p][1], V[p][1]);
+        double vz = fma(sc.hh, A[p][2], V[p][2]);
+        x = fma(hh, r[R_Q1(NM) + 0], x);
+        y = fma(hh, r[R_Q1(NM) + 1], y);
+        z = fma(hh, r[R_Q1(NM) + 2], z);
+        if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
+          const unsigned int slot = atomicAdd(qcount, 1u);
+          q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
+          flag_system(need, 2u, s, slist, qcount + 3);
+          if (carry != nullptr) carry[idx[p]] = 1;  // queued for the next window too
+          act[p] = false;
+        } else {
+          extra = (unsigned)tp_kepler<GUESS>(x, y, z, vx, vy, vz, sc, nnw, stb) - 1u;
+          double xe[3 * NM];
 #pragma unroll
-      for (int i = 0; i < 3 * NM; ++i) xe[i] = rg[R_XE(NM) + i];
-      x = fma(hh, r[R_Q2(NM) + 0], x);
-      y = fma(hh, r[R_Q2(NM) + 1], y);
-      z = fma(hh, r[R_Q2(NM) + 2], z);
-      double ax, ay, az;
-      accel_tp<NM>(x, y, z, xe, prm, sc.eps2, ax, ay, az, far[p]);
-      A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
-      X[p][0] = x; X[p][1] = y; X[p][2] = z;
-      V[p][0] = fma(sc.hh, ax, vx);
-      V[p][1] = fma(sc.hh, ay, vy);
-      V[p][2] = fma(sc.hh, az, vz);
-      ++n_step;
+          for (int i = 0; i < 3 * NM; ++i) xe[i] = rg[R_XE(NM) + i];
+          x = fma(hh, r[R_Q2(NM) + 0], x);
+          y = fma(hh, r[R_Q2(NM) + 1], y);
+          z = fma(hh, r[R_Q2(NM) + 2], z);
+          double ax, ay, az;
+          accel_tp<NM>(x, y, z, xe, prm, sc.eps2, ax, ay, az, far[p]);
+          A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
+          X[p][0] = x; X[p][1] = y; X[p][2] = z;
+          V[p][0] = fma(sc.hh, ax, vx);
+          V[p][1] = fma(sc.hh, ay, vy);
+          V[p][2] = fma(sc.hh, az, vz);
+        }
+      }
+      n_step += wave_count(act[p]);
+      n_newton_extra += wave_sum_rare(extra);
+      if (wave_count(stb != 0) != 0) st_wave |= ST_NEWTON;
     }
   }
+  uint32_t st = 0;
+  unsigned int n_end = 0;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
+    n_end += wave_count(act[p]);
     if (!act[p]) continue;
     tp.x[idx[p]] = X[p][0]; tp.y[idx[p]] = X[p][1]; tp.z[idx[p]] = X[p][2];
     tp.vx[idx[p]] = V[p][0]; tp.vy[idx[p]] = V[p][1]; tp.vz[idx[p]] = V[p][2];
     if (!finite6(X[p][0], X[p][1], X[p][2], V[p][0], V[p][1], V[p][2])) st |= ST_NAN;
   }
+  if ((threadIdx.x & 63) == 0) st |= st_wave;
   if (st) atomicOr(&status[s], st);
   if (stats) {
     unsigned int slots = 0;
 #pragma unroll
     for (int p = 0; p < P; ++p) slots += (p * (int)blockDim.x + (int)threadIdx.x < tile.count) ? 1u : 0u;
-    wave_add(stats + S_TPSTEPS, n_step);
-    wave_add(stats + S_KEPLER, n_step);
-    wave_add(stats + S_NEWTON, n_newton);
-    wave_add(stats + S_DEFER, n_defer);
-    wave_add(stats + S_IDLE, slots * (unsigned)C - n_step);
+    wave_add_uniform(stats + S_TPSTEPS, n_step);
+    wave_add_uniform(stats + S_KEPLER, n_step);
+    wave_add_uniform(stats + S_NEWTON, n_step + n_newton_extra);
+    wave_add_uniform(stats + S_DEFER, n_start - n_end);
+    wave_add_uniform(stats + S_IDLE, wave_sum(slots) * (unsigned)C - n_step);
   }
 }
 
