@@ -145,7 +145,11 @@ StepConst Ctx::step_const() const {
   sc.h = cfg.dt;
   sc.hh = 0.5 * cfg.dt;
   sc.mu = cfg.central_mass;
-  sc.minv = 0.0;
+  // Where 1/r0 <= gate_ir0 the learned guess's gate (dt^2 mu / r0^3 >
+  // KM_TAU_GATE^2) cannot open: the bound is 1e-6 below the exact threshold,
+  // far more than the five roundings of the gate's product can move it.
+  const double gd = sc.mu * (sc.h * sc.h);
+  sc.gate_ir0 = gd > 0.0 ? std::cbrt((KM_TAU_GATE * KM_TAU_GATE) / gd) * (1.0 - 1e-6) : INFINITY;
   sc.eps2 = cfg.softening2;
   sc.tol = cfg.bs_tol;
   return sc;

@@ -83,17 +83,15 @@ struct StepConst {
   double h;     // step
   double hh;    // h/2
   double mu;    // G*M
-  double minv;  // unused slot kept for alignment of the launch argument block
+  double gate_ir0;  // 1/r0 at or below which the learned Kepler guess's gate stays shut
   double eps2;  // pair softening
   double tol;   // BS tolerance
 };
 
-// Square root of a squared distance (a positive normal double well above
-// 2^-767, or zero or infinity): the correctly rounded result by the same
-// reciprocal-square-root refinement the compiler uses for sqrt, without its
-// rescaling of inputs below 2^-767 (which a squared distance in AU never is).
-// Zero and infinity are returned as they are.
-// sqrt_dist2 of a positive finite x: the refinement alone.
+// Square root of a positive finite x well above 2^-767 (a squared distance in
+// AU always is): the correctly rounded result by the same reciprocal-square-
+// root refinement the compiler uses for sqrt, without its rescaling of inputs
+// below 2^-767.  Zero and infinity give NaN.
 __device__ __forceinline__ double sqrt_dist2_pos(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = y * 0.5;
@@ -106,6 +104,8 @@ __device__ __forceinline__ double sqrt_dist2_pos(double x) {
   return fma(d, h, g);
 }
 
+// Square root of a squared distance: sqrt_dist2_pos, with zero and infinity
+// returned as they are.
 __device__ __forceinline__ double sqrt_dist2(double x) {
   const double g = sqrt_dist2_pos(x);
   return (x == 0.0 || x == INFINITY) ? x : g;

@@ -147,12 +147,16 @@ static __device__ __noinline__ double learned_guess_factor(double r0, double eta
 
 // Starting guess of the TP Kepler solve.  GUESS: 0 crude, 1 series, 2 learned
 // (series guess times (1 + c tau^3), c from the MLP, where tau > KM_TAU_GATE).
+// gate_ir0 (StepConst) bounds 1/r0 from below where the gate can open, with a
+// margin of 1e-6 over the rounding of the gate's product, so the one
+// comparison in front of it decides nothing and spares the product for every
+// distant body.
 template <int GUESS>
 __device__ __forceinline__ double tp_guess(const KepStart &k, double mu, double dt,
-                                           const float *__restrict__ nnw) {
+                                           double gate_ir0, const float *__restrict__ nnw) {
   if (GUESS == 0) return guess_crude(k, dt);
   double s = guess_series(k, dt);
-  if (GUESS == 2) {
+  if (GUESS == 2 && k.ir0 > gate_ir0) {
     // gate tau = dt sqrt(mu / r0^3) > KM_TAU_GATE, tested without sqrt or division
     const double ir3 = k.ir0 * (k.ir0 * k.ir0);
     if ((dt * dt) * (mu * ir3) > KM_TAU_GATE * KM_TAU_GATE)
@@ -167,7 +171,7 @@ __device__ __forceinline__ int tp_kepler(double &x, double &y, double &z, double
                                          double &vz, const StepConst &sc,
                                          const float *__restrict__ nnw, uint32_t &st) {
   const KepStart k = kep_start(x, y, z, vx, vy, vz, sc.mu);
-  const double s0 = tp_guess<GUESS>(k, sc.mu, sc.h, nnw);
+  const double s0 = tp_guess<GUESS>(k, sc.mu, sc.h, sc.gate_ir0, nnw);
   bool ok;
   const int it = kepler_drift_from(k, s0, x, y, z, vx, vy, vz, sc.mu, sc.h, ok);
   if (!ok) st |= ST_NEWTON;
