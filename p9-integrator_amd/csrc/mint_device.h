@@ -132,17 +132,9 @@ __device__ __forceinline__ double dot3(double ax, double ay, double az, double b
   return fma(ax, bx, fma(ay, by, az * bz));
 }
 
-// Stumpff functions c0..c3(z) (Danby 1988 §6.9; Stiefel & Scheifele 1971):
-// quarter z until |z| <= 0.1, sum c2 and c3 as 7-term series, then undo the
-// quartering with c0(4z)=2c0^2-1, c1(4z)=c0c1, c2(4z)=c1^2/2,
-// c3(4z)=(c2+c0c3)/4.  Replaces the element round trip and the fixed
-// 8-iteration Kepler-equation solve of src/sim.cuh:90-126,188-324.
-__device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3) {
-  int n = 0;
-  while (fabs(z) > 0.1 && n < 64) {
-    z *= 0.25;
-    ++n;
-  }
+// c0..c3 by the series alone, for |z| <= 0.1.
+__device__ __forceinline__ void stumpff_series(double z, double &c0, double &c1, double &c2,
+                                               double &c3) {
   // Series length by |z| band: the first omitted term is below 3e-19 of the
   // sum in every band (|z| <= 1e-5: degree 2, <= 1e-3: degree 3, else 6).
   // The degree-2 band (every drift of a distant body) is formed first and
@@ -177,6 +169,25 @@ __device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double
   c3 = t3;
   c1 = fma(-z, c3, 1.0);
   c0 = fma(-z, c2, 1.0);
+}
+
+// Stumpff functions c0..c3(z) (Danby 1988 §6.9; Stiefel & Scheifele 1971):
+// quarter z until |z| <= 0.1, sum c2 and c3 as 7-term series, then undo the
+// quartering with c0(4z)=2c0^2-1, c1(4z)=c0c1, c2(4z)=c1^2/2,
+// c3(4z)=(c2+c0c3)/4.  Replaces the element round trip and the fixed
+// 8-iteration Kepler-equation solve of src/sim.cuh:90-126,188-324.
+// |z| <= 0.1 (and NaN) goes to the series directly.
+__device__ __forceinline__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3) {
+  if (!(fabs(z) > 0.1)) {
+    stumpff_series(z, c0, c1, c2, c3);
+    return;
+  }
+  int n = 0;
+  while (fabs(z) > 0.1 && n < 64) {
+    z *= 0.25;
+    ++n;
+  }
+  stumpff_series(z, c0, c1, c2, c3);
   for (; n > 0; --n) {
     const double n3 = fma(c0, c3, c2) * 0.25;
     const double n2 = (c1 * c1) * 0.5;
@@ -197,7 +208,9 @@ struct KepStart {
 __device__ __forceinline__ KepStart kep_start(double x, double y, double z, double vx, double vy,
                                               double vz, double mu) {
   KepStart k;
-  k.r0 = sqrt_dist2(dot3(x, y, z, x, y, z));
+  // r0 = 0 or infinity gives a NaN 1/r0 (so a NaN drift) with either square
+  // root, so the bare refinement is enough
+  k.r0 = sqrt_dist2_pos(dot3(x, y, z, x, y, z));
   k.ir0 = div_normal(1.0, k.r0);
   const double v2 = dot3(vx, vy, vz, vx, vy, vz);
   k.eta0 = dot3(x, y, z, vx, vy, vz);
