@@ -423,9 +423,13 @@ uint32_t bs_drift(std::vector<double> &y, double h, const std::vector<double> &s
       }
     }
     if (conv || depth >= BS_MAXDEPTH) {
-      if (!conv) st |= 4u;
       y = cur;
       t += H;
+      if (!conv) {
+        // halving budget spent: flag, and the rest of the interval in one step
+        st |= 4u;
+        H = h - t;
+      }
     } else {
       H *= 0.5;
       ++depth;

@@ -15,7 +15,7 @@
 namespace mint {
 
 constexpr int KMAX = 8;            // BS rows, substeps N_k = 2^(k+1) = 2..256
-constexpr int BS_MAXDEPTH = 24;    // interval halvings before giving up
+constexpr int BS_MAXDEPTH = 24;    // interval halvings before giving up (then the rest in one step)
 constexpr int NEWTON_MAXIT = 40;   // Kepler Newton iteration cap
 
 constexpr uint32_t ST_NAN = 1u, ST_NEWTON = 2u, ST_BS_DEPTH = 4u, ST_MASSIVE_ENC = 8u;
@@ -734,9 +734,14 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
       }
     }
     if (conv || depth >= BS_MAXDEPTH) {
-      if (!conv) st |= ST_BS_DEPTH;
       for (int i = 0; i < 6; ++i) yv[i] = cur[i];
       t += H;
+      if (!conv) {
+        // the halving budget is spent: flag it and take the rest of the
+        // interval in one step, so the work stays bounded
+        st |= ST_BS_DEPTH;
+        H = h - t;
+      }
     } else {
       H *= 0.5;
       ++depth;
@@ -946,9 +951,14 @@ __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx
       }
     }
     if (conv || depth >= BS_MAXDEPTH) {
-      if (!conv) st |= ST_BS_DEPTH;
       for (int i = 0; i < n; ++i) s[i] = cur[i];
       t += H;
+      if (!conv) {
+        // the halving budget is spent: flag it and take the rest of the
+        // interval in one step, so the work stays bounded
+        st |= ST_BS_DEPTH;
+        H = h - t;
+      }
     } else {
       H *= 0.5;
       ++depth;

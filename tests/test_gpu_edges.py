@@ -46,3 +46,31 @@ def test_single_tp_and_empty_systems():
     print(f"mixed batch: TPs per system {counts}", flush=True)
     rs = _check(st, 50, chunk_steps=16, block_size=64, tps_per_lane=2)
     assert rs["bs_steps"] > 0
+
+
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_BATCHED])
+def test_non_finite_tps_are_flagged_and_isolated(form):
+    """A TP uploaded with a NaN position and one with an infinite velocity:
+    their systems are flagged non-finite (as the oracle flags them), and every
+    other TP and every massive body ends bit for bit where it ends without
+    them (the encounter check never fires on a NaN or infinite separation, and
+    every loop a non-finite state can reach is bounded)."""
+    st = W.p9_sweep(systems=[0, 1, 2], tp_scale=1 / 32)
+    bad = st.copy()
+    i_nan = int(st.tp_offset[0]) + 3
+    i_inf = int(st.tp_offset[2]) + 5
+    bad.tx[i_nan, 0] = np.nan
+    bad.tv[i_inf, 1] = np.inf
+    kw = dict(chunk_steps=16, block_size=64, tps_per_lane=1) if form == mint.FORM_BATCHED else {}
+    print(f"non-finite TPs {i_nan}, {i_inf}; form {form}", flush=True)
+    out_c, stat_c, _ = run_gpu(st, 40, 0.5, form, mint.GUESS_SERIES, **kw)
+    out_b, stat_b, _ = run_gpu(bad, 40, 0.5, form, mint.GUESS_SERIES, **kw)
+    keep = np.ones(st.n_tp, bool)
+    keep[[i_nan, i_inf]] = False
+    assert np.array_equal(out_b.tx[keep], out_c.tx[keep]) and np.array_equal(out_b.tv[keep], out_c.tv[keep])
+    assert np.array_equal(out_b.mx, out_c.mx) and np.array_equal(out_b.mv, out_c.mv)
+    assert not np.any(stat_c & mint.ST_NAN)
+    assert stat_b[0] & mint.ST_NAN and stat_b[2] & mint.ST_NAN and not stat_b[1] & mint.ST_NAN
+    assert not np.all(np.isfinite(out_b.tx[i_nan])) and not np.all(np.isfinite(out_b.tv[i_inf]))
+    _, rst, _ = O.run(bad, 40, 0.5, guess=1)
+    assert np.array_equal(rst & mint.ST_NAN, stat_b & mint.ST_NAN)

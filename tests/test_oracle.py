@@ -221,6 +221,27 @@ def test_bs_close_encounter_converges_with_tolerance():
     assert np.max(np.abs(y2[:3] - xa)) > 1e-6
 
 
+def test_bs_spent_halving_budget_is_flagged_and_bounded():
+    # A tolerance no error meets (negative; the library refuses it, the
+    # oracle's single-drift entry does not): the 24 halvings run out, the
+    # drift is flagged (status bit 4) and the rest of the interval is taken in
+    # one step.  At most 24 failed attempts, the accepted one and the rest
+    # (plus a last sliver from rounding of t), each at most 1 + 2 + ... + 256
+    # = 511 right-hand sides; carrying on at h / 2^24 would take millions.
+    x = np.array([30.0, 0.3, 0.0])
+    v = np.array([0.0, 0.18, 0.01])
+    px = np.array([[30.0, 0.0, 0.0]])
+    pv = np.array([[0.0, 0.1826, 0.0]])
+    y, st, n = O.bs_tp(np.concatenate([x, v]), [5.15e-5], [2.33], px, pv, 0.5, tol=-1.0)
+    assert st & 4
+    assert n <= 28 * 511, n
+    assert np.all(np.isfinite(y))
+    # the last row of a full-interval attempt is still a fair estimate
+    yc, stc, _ = O.bs_tp(np.concatenate([x, v]), [5.15e-5], [2.33], px, pv, 0.5, tol=1e-13)
+    assert stc == 0
+    assert np.max(np.abs(y[:3] - yc[:3])) <= 1e-6 * np.linalg.norm(yc[:3])
+
+
 # --- whole step -------------------------------------------------------------------
 class _St:
     def __init__(self, m, mx, mv, tx=None, tv=None):
