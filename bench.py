@@ -34,7 +34,7 @@ N_M = 5
 # HBM bytes per tp_main_kernel launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (calibrated by a 1 GiB copy: FETCH x2.0, WRITE x1.0), per launch shape
 # (block, TPs per lane, window); profiles/tp_main_kernel_p9sweep4096.md
-MEASURED_TRAFFIC = {(128, 1, 128): 910.8e6}
+MEASURED_TRAFFIC = {(256, 1, 128): 959.4e6}
 
 
 def flops_per_tp_step(nbar: float, nm: int = N_M) -> float:
