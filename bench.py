@@ -252,8 +252,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (systems would shard "
-                                   "across GPUs with no exchange, see the sharding paragraph of DESIGN.md)"}))
+        print(json.dumps({"error": "one GPU only: --gpus must be 1 (independent systems would shard "
+                                   "across GPUs by system range, with no exchange between them)"}))
         sys.exit(2)
     if args.steps is None:
         args.steps = 10000 if args.workload == "embryos" else 2000
