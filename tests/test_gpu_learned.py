@@ -97,3 +97,27 @@ def test_scorer_prebucketing_is_bitwise_neutral():
             stats.append(g.stats())
     assert stats[1]["prebucketed"] > 0
     assert np.array_equal(outs[0].tx, outs[1].tx) and np.array_equal(outs[0].tv, outs[1].tv)
+
+
+@pytest.mark.parametrize("nm", [1, 2, 3, 4, 6, 7, 8])
+def test_scorer_at_every_massive_count_matches_the_oracle(nm):
+    """The trained scorer's queue head at 1..8 massive bodies (its pair loop
+    and the pre-bucket kernel are instantiated per count): TPs around the
+    first and last planet are picked, and every TP and body still ends bit
+    for bit where the oracle puts it."""
+    from gpu_cases import planet_cloud
+
+    st = planet_cloud(nm, n_tp=128, seed=3)
+    print(f"scorer, planet cloud nm={nm}", flush=True)
+    with mint.Mint(dt=0.5, use_scorer=1, collect_stats=1, chunk_steps=16, block_size=64, tps_per_lane=1) as g:
+        _weights(g)
+        g.upload(st)
+        g.step(40)
+        out = g.download()
+        s = g.stats()
+        status = g.status()
+    ref, rst, rs = O.run(st, 40, 0.5, guess=1)
+    assert s["prebucketed"] > 0
+    assert s["bs_steps"] == rs["bs_steps"]
+    assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
+    assert np.array_equal(out.mx, ref["mx"]) and np.all(status == rst)
