@@ -3,6 +3,8 @@ sweep (dt = 0.5 yr), one MI355X, through libmint's C ABI.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--form learned|batched|flat|plain]
   python bench.py --workload embryos [--steps K]     (the 1000-embryo all-massive system)
+  python bench.py --workload p9_single [--steps K]   (one P9 system with 1e5 test particles)
+  python bench.py --workload earth_mars [--steps K]  (Sun + Earth + Mars: energy against the oracle)
 
 A "step" is one Mercurius step of every system of the sweep (about 5.6 M test
 particles, five massive bodies each).  The state is uploaded to HBM before the
@@ -121,7 +123,18 @@ def cpu_baseline(st_full, budget_s=12.0):
     return dict(value=stats["tp_steps"] / dt, unit="TP-steps/s", cores=1, kind="port",
                 sample=f"systems 0-{nsys - 1} of the sweep ({sample.n_tp} TPs), {steps} steps, oracle single thread, "
                        f"{dt:.1f} s",
-                all_cores=dict(value=stats_n["tp_steps"] / dtn, cores=ncores, seconds=round(dtn, 2)))
+                all_cores=dict(value=stats_n["tp_steps"] / dtn, cores=ncores, seconds=round(dtn, 2)),
+                cpu=cpu_model())
+
+
+def cpu_model() -> str:
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            names = [ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else "unknown"
+    except OSError:
+        return "unknown"
 
 
 def copy_probe(nbytes):
@@ -137,6 +150,74 @@ def copy_probe(nbytes):
         assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
     assert hip.hipDeviceSynchronize() == 0
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
+def bench_p9_single(args):
+    """One P9 system with 1e5 test particles (dt 0.5 yr): the batched form at
+    its autotuned shape, the plain form (a launch per step) on fewer steps, and
+    the oracle on one core for 100 steps of the same system."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # test infrastructure: the CPU baseline leg only
+
+    st = W.p9_single()
+    ntp = st.n_tp
+    g = make_ctx(mint.FORM_BATCHED, mint.GUESS_SERIES, 0)
+    g.upload(st)
+    shape = g.autotune(args.tune_cache)
+    wall, ev_s, E0, E1 = timed_run(g, st, args.steps, args.warmup)
+    status = int(g.status()[0])
+    g.close()
+    value = ntp * args.steps / ev_s
+    log(f"bench: p9_single batched {shape}: {args.steps} steps in {ev_s * 1e3:.1f} ms -> {value:.4e} TP-steps/s")
+    ps = min(args.steps, 1000)
+    gpl = make_ctx(mint.FORM_PLAIN, mint.GUESS_CRUDE, 0)
+    _, pe, _, _ = timed_run(gpl, st, ps, 2)
+    gpl.close()
+    plain = ntp * ps / pe
+    t0 = time.perf_counter()
+    _, _, so = O.run(st, 100, 0.5, guess=1, threads=1)
+    cpu = so["tp_steps"] / (time.perf_counter() - t0)
+    out = {"metric": "test-particle steps/s, one P9 system with 1e5 TPs", "value": value, "unit": "TP-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ev_s * 1e3 / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "p9_single_1e5", "tp_total": ntp, "massive": 5, "dt_yr": 0.5,
+                      "form": "batched", "launch_shape": shape},
+           "forms_tp_steps_per_s": {"batched": value, "plain": plain},
+           "cpu_baseline": {"value": cpu, "unit": "TP-steps/s", "cores": 1, "kind": "port",
+                            "sample": f"the same system, 100 steps, oracle single thread", "cpu": cpu_model()},
+           "energy": {"max_rel_dE_over_run": float(abs((E1[0] - E0[0]) / E0[0])), "status": status}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_earth_mars(args):
+    """Sun, Earth and Mars (dt 0.001 yr): the energy check of the plain step,
+    the GPU's run against the oracle's over the same steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # test infrastructure: checker only
+
+    st = W.earth_mars()
+    g = mint.Mint(dt=0.001)
+    g.upload(st)
+    E0 = g.energy()
+    g.reset_counters()
+    g.step(args.steps)
+    ev_s = g.timing()["ms_total"] / 1e3
+    E1 = g.energy()
+    fin = g.download()
+    g.close()
+    t0 = time.perf_counter()
+    ref, _, _ = O.run(st, args.steps, 0.001, guess=1)
+    cpu_s = time.perf_counter() - t0
+    Eo = O.energy(st.m, ref["mx"], ref["mv"])
+    out = {"metric": "energy check, Sun + Earth + Mars, dt 0.001 yr", "steps": args.steps,
+           "gpu_seconds": ev_s, "oracle_seconds_1_core": cpu_s,
+           "max_rel_dE_gpu": float(abs((E1[0] - E0[0]) / E0[0])),
+           "max_rel_dE_oracle": float(abs((Eo[0] - E0[0]) / E0[0])),
+           "rel_E_gpu_minus_oracle": float(abs((E1[0] - Eo[0]) / E0[0])),
+           "massive_state_bitwise_equal_to_oracle": bool(np.array_equal(fin.mx, ref["mx"]) and
+                                                         np.array_equal(fin.mv, ref["mv"])),
+           "cpu": cpu_model()}
+    print(json.dumps(out), flush=True)
 
 
 EMBRYO_METRIC = "massive body-steps/s, 1000-embryo system (all massive, O(N^2) kicks), dt 0.002"
@@ -243,8 +324,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "p9-integrator_amd", "autotune.cache"))
     ap.add_argument("--shape", default=None, help="block,tps_per_lane,window: skip the autotuner")
-    ap.add_argument("--workload", default="p9_sweep", choices=["p9_sweep", "embryos"],
-                    help="embryos: the 1000-embryo all-massive system (default steps 10000)")
+    ap.add_argument("--workload", default="p9_sweep", choices=["p9_sweep", "embryos", "p9_single", "earth_mars"],
+                    help="embryos: the 1000-embryo all-massive system (default steps 10000); p9_single: one "
+                         "P9 system with 1e5 TPs (10000); earth_mars: the energy check (100000)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -256,9 +338,15 @@ def main():
                                    "across GPUs by system range, with no exchange between them)"}))
         sys.exit(2)
     if args.steps is None:
-        args.steps = 10000 if args.workload == "embryos" else 2000
+        args.steps = {"embryos": 10000, "p9_single": 10000, "earth_mars": 100000}.get(args.workload, 2000)
     if args.workload == "embryos":
         bench_embryos(args)
+        return
+    if args.workload == "p9_single":
+        bench_p9_single(args)
+        return
+    if args.workload == "earth_mars":
+        bench_earth_mars(args)
         return
 
     log(f"bench: generating the {args.systems}-system sweep")
