@@ -478,10 +478,10 @@ __device__ __forceinline__ float scorer_best(const float (*f)[SC_IN], const floa
   return best;
 }
 
-// One TP per lane: registers are capped for four waves per SIMD (128 VGPRs,
-// ~30 spilled to scratch), which measures 2-3 % faster than three waves with
-// no spills (five waves spill ~65 and run 40 % slower).  Two or four TPs per
-// lane keep the compiler's allocation.
+// One TP per lane: registers are capped for four waves per SIMD (128 VGPRs;
+// the few that spill are off the step loop's common path), which measured
+// faster than three waves with no spills (five waves spill ~50-65 and ran
+// 40 % slower).  Two or four TPs per lane keep the compiler's allocation.
 template <int NM, int P, int GUESS>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 : 1))) tp_main_kernel(
     const Tile *__restrict__ tiles, int ntiles, int tiles_per_xcd, TPArrays tp,
