@@ -687,6 +687,35 @@ __device__ __forceinline__ double bs_fac(int j) {
 // The extrapolation tableau element (row, i) lives at Rt[(row * 6 + i) * rs]:
 // a private array (rs = 1) or a lane's column of an LDS block (rs = lanes).
 // G > 1: the G lanes of a group run the same step together (see rhs_tp).
+//
+// bs_extrapolate: row k of the extrapolation.  cur holds row k's Gragg value
+// on entry and T_kk on return; the tableau is updated in place.  Returns the
+// scaled error |T_kk - T_k,k-1| (0 for k = 0).
+__device__ __forceinline__ double bs_extrapolate(int k, double *cur, double *Rt, int rs, double xs,
+                                                 double vs) {
+  for (int j = 1; j <= k; ++j) {
+    const double fac = bs_fac(j);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double *slot = Rt + ((j - 1) * 6 + i) * rs;
+      const double prev = *slot;
+      *slot = cur[i];
+      cur[i] = cur[i] + div_normal(cur[i] - prev, fac);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) Rt[(k * 6 + i) * rs] = cur[i];
+  double err = 0.0;
+  if (k >= 1) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double e = div_normal(fabs(cur[i] - Rt[((k - 1) * 6 + i) * rs]), i < 3 ? xs : vs);
+      err = e > err ? e : err;
+    }
+  }
+  return err;
+}
+
 template <int NM, int G = 1>
 __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const double *__restrict__ Y,
                                                 const double *__restrict__ W,
@@ -708,29 +737,10 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
       gragg_tp<NM, G>(yv, f0, t, H, 2 << k, Y, W, P4, h, prm, mu, eps2, cur, nrhs, jl, base);
-      for (int j = 1; j <= k; ++j) {
-        const double fac = bs_fac(j);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          double *slot = Rt + ((j - 1) * 6 + i) * rs;
-          const double prev = *slot;
-          *slot = cur[i];
-          cur[i] = cur[i] + div_normal(cur[i] - prev, fac);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 6; ++i) Rt[(k * 6 + i) * rs] = cur[i];
-      if (k >= 1) {
-        double err = 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          const double e = div_normal(fabs(cur[i] - Rt[((k - 1) * 6 + i) * rs]), i < 3 ? xs : vs);
-          err = e > err ? e : err;
-        }
-        if (err <= tol) {
-          conv = true;
-          break;
-        }
+      const double err = bs_extrapolate(k, cur, Rt, rs, xs, vs);
+      if (k >= 1 && err <= tol) {
+        conv = true;
+        break;
       }
     }
     if (conv || depth >= BS_MAXDEPTH) {
@@ -739,6 +749,111 @@ __device__ __forceinline__ uint32_t bs_drift_tp(double *yv, double h, const doub
       if (!conv) {
         // the halving budget is spent: flag it and take the rest of the
         // interval in one step, so the work stays bounded
+        st |= ST_BS_DEPTH;
+        H = h - t;
+      }
+    } else {
+      H *= 0.5;
+      ++depth;
+    }
+  }
+  return st;
+}
+
+// bs_drift_tp for an entry served by R row groups of GP lanes each (lanes
+// ebase .. ebase + R GP - 1; this lane is perturber lane jl of row group
+// row): the Gragg rows kb .. kb+R-1 of one attempt run side by side, one per
+// group, in lockstep by substep, and as each row completes every lane of the
+// entry takes it from its group (a shuffle), extends the extrapolation and
+// tests convergence, leaving at the first converged row.  Every row and every
+// extrapolation step is formed as in bs_drift_tp and in the same order, so
+// the result and the right-hand-side count (rows up to the converged one) are
+// the same; the latency of an attempt drops from 2 + 4 + ... + N_k substeps
+// to N_k.
+template <int NM, int GP, int R>
+__device__ __forceinline__ uint32_t bs_drift_tp_rows(double *yv, double h,
+                                                     const double *__restrict__ Y,
+                                                     const double *__restrict__ W,
+                                                     const double *__restrict__ P4,
+                                                     const double *__restrict__ prm, double mu,
+                                                     double eps2, double tol, int &nrhs,
+                                                     double *Rt, int rs, int jl, int row, int ebase) {
+  const int base = ebase + row * GP;  // this row group's first lane
+  const int j = jl < NM ? jl : NM - 1;
+  uint32_t st = 0;
+  double t = 0.0, H = h;
+  int depth = 0;
+  while (t < h) {
+    if (H > h - t) H = h - t;
+    double f0[6];
+    rhs_tp<NM, GP>(t, yv, Y, W, P4, h, prm, mu, eps2, f0, jl, base);
+    nrhs += 1;
+    const double xs = sqrt_dist2(dot3(yv[0], yv[1], yv[2], yv[0], yv[1], yv[2]));
+    const double vs = sqrt_dist2(dot3(yv[3], yv[4], yv[5], yv[3], yv[4], yv[5]));
+    double cur[6];
+    bool conv = false;
+    for (int kb = 0; kb < KMAX && !conv; kb += R) {
+      const int klast = kb + R - 1 < KMAX - 1 ? kb + R - 1 : KMAX - 1;
+      const int km = kb + row;                        // this group's row
+      const int Nm = km <= klast ? (2 << km) : 0;     // 0: no row for this group
+      const int Nmax = 2 << klast;
+      const double hs = Nm > 0 ? H / (double)Nm : 0.0;
+      const double h2 = 2.0 * hs;
+      double z0[6], z1[6], f[6], mine[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        z0[i] = yv[i];
+        z1[i] = fma(hs, f0[i], yv[i]);
+        mine[i] = 0.0;
+      }
+      // the lane's perturber one substep ahead, as in gragg_tp
+      double px = 0.0, py = 0.0, pz = 0.0;
+      if (Nm > 0) {
+        const double t1 = t + hs;
+        perturber_pos<NM>(j, t1, cache_point(t1, h), Y, W, P4, mu, px, py, pz);
+      }
+      int k = kb;  // next row to extrapolate
+      for (int m = 1; m <= Nmax; ++m) {
+        // one right-hand side for every row still running (the rows that
+        // finish at this substep included), then each row's own update
+        if (m <= Nm) {
+          double nx = 0.0, ny = 0.0, nz = 0.0;
+          if (m < Nm) {
+            const double tn = m + 1 < Nm ? t + (double)(m + 1) * hs : t + H;
+            perturber_pos<NM>(j, tn, cache_point(tn, h), Y, W, P4, mu, nx, ny, nz);
+          }
+          rhs_tp_at<NM>(z1, px, py, pz, prm, mu, eps2, f, jl, base);
+          if (m < Nm) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+              const double z2 = fma(h2, f[i], z0[i]);
+              z0[i] = z1[i];
+              z1[i] = z2;
+            }
+            px = nx;
+            py = ny;
+            pz = nz;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) mine[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
+          }
+        }
+        if (m == (2 << k)) {  // row k has just been completed by its group
+          const int src = ebase + (k - kb) * GP;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) cur[i] = __shfl(mine[i], src, 64);
+          nrhs += 2 << k;
+          const double err = bs_extrapolate(k, cur, Rt, rs, xs, vs);
+          if (k >= 1 && err <= tol) conv = true;
+          ++k;
+          if (conv) break;
+        }
+      }
+    }
+    if (conv || depth >= BS_MAXDEPTH) {
+      for (int i = 0; i < 6; ++i) yv[i] = cur[i];
+      t += H;
+      if (!conv) {
         st |= ST_BS_DEPTH;
         H = h - t;
       }

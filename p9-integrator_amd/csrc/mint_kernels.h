@@ -679,9 +679,9 @@ __global__ void __launch_bounds__(256) tp_prebucket_kernel(
 // ---------------------------------------------------------------------------
 // BS queue consumer.  The queue is short (tens to thousands of entries) and
 // each entry is a sequential chain of up to C steps.  G lanes serve one entry
-// (G = 1, or G = 8 with each BS right-hand side split over the perturbers,
-// for the shortest chain); wave w takes entries w*L .. w*L+L-1, one per
-// group of G lanes.  Spread, the entries go over every wave of the grid
+// (G = 1, or, for the shortest chain, G = 4 GP: each BS right-hand side split
+// over GP perturber lanes and four extrapolation rows run side by side); wave
+// w takes entries w*L .. w*L+L-1, one per group of G lanes.  Spread, the entries go over every wave of the grid
 // (L = ceil(count / waves)); dense, beside a long main kernel, they are
 // packed 64 / G to a wave, so that few SIMDs give up register space to this
 // kernel (322 registers a lane) and the main kernel keeps its occupancy.
@@ -709,8 +709,14 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned int lane = threadIdx.x & 63u;
   constexpr unsigned int GPW = 64u / (unsigned int)G;  // groups per wave
-  const unsigned int g = lane / (unsigned int)G, jl = lane % (unsigned int)G, base = lane - jl;
-  const unsigned int lead = jl == 0u ? 1u : 0u;
+  // G > 1: GP lanes per BS row (one per massive body, rounded up to a power
+  // of two) times R row groups that run the extrapolation rows side by side
+  constexpr int GP = G == 1 ? 1 : (NM <= 1 ? 1 : (NM <= 2 ? 2 : (NM <= 4 ? 4 : 8)));
+  constexpr int R = G / GP;
+  static_assert(G == 1 || G == GP * R, "G is GP lanes times R row groups");
+  const unsigned int g = lane / (unsigned int)G, el = lane % (unsigned int)G, ebase = lane - el;
+  const unsigned int jl = el % (unsigned int)GP, row = el / (unsigned int)GP;
+  const unsigned int lead = el == 0u ? 1u : 0u;
   unsigned int L = dense ? GPW : (n + nw - 1u) / nw;
   L = L < 1u ? 1u : (L > GPW ? GPW : L);
   for (unsigned int e = w * L + g; g < L && e < n; e += nw * L) {
@@ -746,9 +752,14 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
-        st |= bs_drift_tp<NM, G>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
-                                 pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol,
-                                 nr, Rt, 64, (int)jl, (int)base);
+        if (G == 1)
+          st |= bs_drift_tp<NM, 1>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
+                                   pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol,
+                                   nr, Rt, 64);
+        else
+          st |= bs_drift_tp_rows<NM, GP, R>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
+                                            pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2,
+                                            sc.tol, nr, Rt, 64, (int)jl, (int)row, (int)ebase);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
         n_bs += lead;
         n_rhs += lead * (unsigned)nr;

@@ -79,8 +79,8 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
 }
 
 // lanes per queue entry in the grouped encounter kernel: one per perturber,
-// rounded up to a power of two
-constexpr int ENC_G = NMV <= 1 ? 1 : (NMV <= 2 ? 2 : (NMV <= 4 ? 4 : 8));
+// rounded up to a power of two, times four row groups of the BS extrapolation
+constexpr int ENC_G = 4 * (NMV <= 1 ? 1 : (NMV <= 2 ? 2 : (NMV <= 4 ? 4 : 8)));
 
 template <int G>
 hipError_t encounter_g(hipStream_t st, int guess, int grid, const QEntry *q, const unsigned int *qcount,
