@@ -483,14 +483,26 @@ __global__ void __launch_bounds__(MB_BLOCK)
 }
 
 // ---------------------------------------------------------------------------
-// Bulirsch-Stoer drift of one encounter group by one wavefront: lane a holds
-// member a's state (6 doubles), its mass and changeover fields, and its
-// tableau; each right-hand side reads the other members' positions and fields
-// by wavefront shuffles.  Same operations, per member and component, as the
-// single-lane group BS; the error is the max over members, which does not
-// depend on the order it is taken in.
-__device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu, double eps2,
+// Bulirsch-Stoer drift of one encounter group by one wavefront.  The wave is
+// split into row groups of GP lanes (GP: the group size rounded up to a power
+// of two); lane a of a row group holds member a's state (6 doubles), its mass
+// and changeover fields, and each right-hand side reads the other members of
+// its row group by wavefront shuffles.  The Gragg rows of an attempt run side
+// by side, rows kb .. kb+R-1 one per row group (R = min(64 / GP, KMAX)), in
+// lockstep by substep; as row k completes, every lane takes member a's row-k
+// value from that row group and extends its own copy of the extrapolation
+// tableau, so the tableau, the error (the max over members, which does not
+// depend on the order it is taken in) and the convergence test are the same
+// in every row group.  Every row and extrapolation step is formed as in the
+// single-lane group BS and in the same order (same bits and right-hand-side
+// counts); the latency of an attempt drops from 2 + 4 + ... + N_k substeps to
+// N_k while k < R.  Once the halving budget is spent the rest of the interval
+// is taken in one step and flagged, as in the single-lane group BS.
+__device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, double mu, double eps2,
                                   double tol, double mj, double loj, double wvj, unsigned int &nrhs) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const int row = lane / GP, base = row * GP;
+  const int R = 64 / GP < KMAX ? 64 / GP : KMAX;
   const bool act = a < gs;
   auto rhs = [&](const double *z, double *f) {
     const double x = z[0], yy = z[1], zz = z[2];
@@ -502,8 +514,9 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
     const double w0 = div_normal(mu, r2 * r);
     double ax = -(w0 * x), ay = -(w0 * yy), az = -(w0 * zz);
     for (int b = 0; b < gs; ++b) {
-      const double xb = __shfl(x, b, 64), yb = __shfl(yy, b, 64), zb = __shfl(zz, b, 64);
-      const double mb = __shfl(mj, b, 64), lob = __shfl(loj, b, 64), wvb = __shfl(wvj, b, 64);
+      const int src = base + b;
+      const double xb = __shfl(x, src, 64), yb = __shfl(yy, src, 64), zb = __shfl(zz, src, 64);
+      const double mb = __shfl(mj, src, 64), lob = __shfl(loj, src, 64), wvb = __shfl(wvj, src, 64);
       if (b == a) continue;
       const double dx = xb - x, dy = yb - yy, dz = zb - zz;
       const double d2 = dot3(dx, dy, dz, dx, dy, dz);
@@ -526,49 +539,68 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
     ++nrhs;
     const double scx = sqrt_dist2(dot3(y[0], y[1], y[2], y[0], y[1], y[2]));
     const double scv = sqrt_dist2(dot3(y[3], y[4], y[5], y[3], y[4], y[5]));
-    double R[KMAX][6], cur[6];
+    double Rt[KMAX][6], cur[6], z0[6], z1[6], mine[6];
+    double hs = 0.0, h2 = 0.0;
+    int Nm = 0, m = 0, kb = 0;
     bool conv = false;
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-      const int Ns = 2 << k;
-      const double hs = H / (double)Ns, h2 = 2.0 * hs;
-      double z0[6], z1[6], f[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        z0[c] = y[c];
-        z1[c] = fma(hs, f0[c], y[c]);
-      }
-      for (int m = 1; m < Ns; ++m) {
-        rhs(z1, f);
+      if (k == kb) {  // rows kb .. kb+R-1 start, one per row group
+        const int km = kb + row;
+        Nm = (row < R && km < KMAX) ? (2 << km) : 0;  // 0: no row for this group
+        hs = Nm > 0 ? H / (double)Nm : 0.0;
+        h2 = 2.0 * hs;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          const double z2 = fma(h2, f[c], z0[c]);
-          z0[c] = z1[c];
-          z1[c] = z2;
+          z0[c] = y[c];
+          z1[c] = fma(hs, f0[c], y[c]);
+          mine[c] = 0.0;
+        }
+        m = 0;
+      }
+      // substeps up to row k's last: one right-hand side for every row still
+      // running, then each row's own update (a row group's lanes share a row)
+      const int Ns = 2 << k;
+      while (m < Ns) {
+        ++m;
+        if (m <= Nm) {
+          double f[6];
+          rhs(z1, f);
+          if (m < Nm) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+              const double z2 = fma(h2, f[c], z0[c]);
+              z0[c] = z1[c];
+              z1[c] = z2;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) mine[c] = 0.5 * (z1[c] + fma(hs, f[c], z0[c]));
+          }
         }
       }
-      rhs(z1, f);
-      nrhs += (unsigned int)Ns;
+      const int src = (k - kb) * GP + a;  // member a in the row group that ran row k
 #pragma unroll
-      for (int c = 0; c < 6; ++c) cur[c] = 0.5 * (z1[c] + fma(hs, f[c], z0[c]));
+      for (int c = 0; c < 6; ++c) cur[c] = __shfl(mine[c], src, 64);
+      nrhs += (unsigned int)Ns;
 #pragma unroll
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          const double prev = R[j - 1][c];
-          R[j - 1][c] = cur[c];
+          const double prev = Rt[j - 1][c];
+          Rt[j - 1][c] = cur[c];
           cur[c] = cur[c] + div_normal(cur[c] - prev, fac);
         }
       }
 #pragma unroll
-      for (int c = 0; c < 6; ++c) R[k][c] = cur[c];
+      for (int c = 0; c < 6; ++c) Rt[k][c] = cur[c];
       if (k >= 1) {
         double err = 0.0;
         if (act) {
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
-            const double e = div_normal(fabs(cur[c] - R[k - 1][c]), c < 3 ? scx : scv);
+            const double e = div_normal(fabs(cur[c] - Rt[k - 1][c]), c < 3 ? scx : scv);
             err = e > err ? e : err;
           }
         }
@@ -582,12 +614,18 @@ __device__ uint32_t bs_group_wave(double *y, int a, int gs, double h, double mu,
           break;
         }
       }
+      if (k == kb + R - 1) kb += R;
     }
     if (conv || depth >= BS_MAXDEPTH) {
-      if (!conv) st |= ST_BS_DEPTH;
 #pragma unroll
       for (int c = 0; c < 6; ++c) y[c] = cur[c];
       t += H;
+      if (!conv) {
+        // the halving budget is spent: flag it and take the rest of the
+        // interval in one step, so the work stays bounded
+        st |= ST_BS_DEPTH;
+        H = h - t;
+      }
     } else {
       H *= 0.5;
       ++depth;
@@ -654,12 +692,15 @@ __global__ void __launch_bounds__(64)
     }
   }
   const BP bp = bp_of(bp_all, s, N);
-  const int a = threadIdx.x;
+  const int lane = (int)threadIdx.x;
   unsigned int nrhs = 0;
   uint32_t st = 0;
   for (int g = (int)blockIdx.x - nK; g < G; g += nG) {
     const int g0 = gst[g], gs = gst[g + 1] - g0;
     if (gs > MB_GMAX) continue;
+    int GP = 1;  // lanes per row group: the group size rounded up to a power of two
+    while (GP < gs) GP <<= 1;
+    const int a = lane & (GP - 1);  // member of this lane in its row group
     const int j = a < gs ? gm[g0 + a] : -1;
     // lanes past the group carry a unit position so their own (unused)
     // Sun term stays finite
@@ -674,14 +715,14 @@ __global__ void __launch_bounds__(64)
       loj = bp.lo[j];
       wvj = bp.winv[j];
     }
-    st |= bs_group_wave(y, a, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
-    if (j >= 0)
+    st |= bs_group_wave(y, a, GP, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
+    if (j >= 0 && lane < GP)  // every row group ends with the same state; the first writes it
       for (int c = 0; c < 3; ++c) {
         XB[3 * j + c] = y[c];
         VB[3 * j + c] = y[3 + c];
       }
   }
-  if (st && a == 0) atomicOr(status + s, st);
+  if (st && lane == 0) atomicOr(status + s, st);
 }
 
 size_t close_lds(const MBShape &sh) {

@@ -57,6 +57,24 @@ def test_embryo_ring_bitwise(n, steps):
     assert np.array_equal(E, O.energy(st.m, ref["mx"], ref["mv"]))
 
 
+@pytest.mark.parametrize("n,span,steps,mean_size", [(30, 0.3, 40, 4), (48, 0.25, 40, 15), (12, 0.02, 100, 12),
+                                                    (24, 0.06, 100, 24), (40, 0.08, 20, 40), (64, 0.1, 20, 64)])
+def test_every_group_lane_layout_bitwise(n, span, steps, mean_size):
+    """Encounter groups in every lane layout of the wavefront group BS: groups
+    of 2..8 members (all eight Gragg rows side by side), 12 (rows four at a
+    time), 24 (two at a time), 40 and 64 members (one row at a time), and a
+    mix of sizes; bit for bit against the oracle."""
+    st = embryo_cluster(n, span=span)
+    print(f"embryo cluster n={n} span={span} steps={steps}", flush=True)
+    ref, rst, rstats = O.run(st, steps, 0.002, bs_tol=1e-11, radii=st.meta["radius"])
+    out, status, stats, _ = _gpu(st, steps, 0.002, radii=st.meta["radius"], bs_tol=1e-11)
+    assert rstats["massive_group_bodies"] >= mean_size * rstats["massive_groups"] > 0
+    _same(out, ref)
+    assert np.array_equal(status, rst)
+    for k in ("massive_groups", "massive_group_bodies", "overlaps"):
+        assert stats[k] == rstats[k], k
+
+
 def test_lane_engine_and_many_body_engine_agree_on_8_bodies():
     st = embryo_cluster(8, seed=0)
     ref, rst, rstats = O.run(st, 400, 0.002, bs_tol=1e-11)
