@@ -104,6 +104,14 @@ __device__ __forceinline__ void fold3(const double *p, int n, int stride, double
   }
 }
 
+// Staged partner windows in LDS carry one double of padding after each
+// 16-body chunk.  The 16 chunk slots of a wavefront read the same offset in 16
+// consecutive chunks; unpadded, those chunks start 48 (positions) or 16 (per-
+// body scalars) doubles apart, on only two LDS bank positions.  With chunk
+// strides of 49 and 17 doubles the 16 chunks start on 16 distinct even banks.
+__device__ __forceinline__ int lds3(int k) { return 3 * k + (k >> 4); }  // position x of window body k
+__device__ __forceinline__ int lds1(int k) { return k + (k >> 4); }      // scalar of window body k
+
 // Q = (sum_j m_j v_j)/M by the whole workgroup: chunk sums (from zero, index
 // order) in part[], then added in chunk order by one thread.
 __device__ void wg_momentum(const double *__restrict__ V, const double *__restrict__ m, int N,
@@ -167,10 +175,13 @@ __global__ void __launch_bounds__(MB_BLOCK)
   const double *VB = vb_all + (size_t)s * N * 3;
   const int WB = sh.CS * MB_CHUNK;                   // bodies per staged window
   double *part = lds;                                // [nch][TI][3]
-  double *qpart = part + (size_t)sh.nch * sh.TI * 3;  // [nch][3]
-  double *Qs = qpart + 3 * sh.nch;                   // [3]
-  double *sx = Qs + 3;                               // [WB][3] partner positions
-  double *sm = sx + 3 * WB, *slo = sm + WB, *swv = slo + WB;
+  // momentum chunk sums [nch][3]: read before the barrier that opens the first
+  // window, so they share part's space (part is written after it)
+  double *qpart = part;
+  double *Qs = part + (size_t)sh.nch * sh.TI * 3;    // [3]
+  double *sx = Qs + 3;                               // partner positions, lds3 layout
+  double *sm = sx + 3 * WB + sh.CS;                  // per-body scalars, lds1 layout
+  double *slo = sm + WB + sh.CS, *swv = slo + WB + sh.CS;
   const double *X = INIT ? XA : XB;
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (!INIT) {
@@ -206,12 +217,12 @@ __global__ void __launch_bounds__(MB_BLOCK)
         yk = fma(sc.hh, q1, yk);
         zk = fma(sc.hh, q2, zk);
       }
-      sx[3 * k + 0] = xk;
-      sx[3 * k + 1] = yk;
-      sx[3 * k + 2] = zk;
-      sm[k] = bp.m[kk];
-      slo[k] = bp.lo[kk];
-      swv[k] = bp.winv[kk];
+      sx[lds3(k) + 0] = xk;
+      sx[lds3(k) + 1] = yk;
+      sx[lds3(k) + 2] = zk;
+      sm[lds1(k)] = bp.m[kk];
+      slo[lds1(k)] = bp.lo[kk];
+      swv[lds1(k)] = bp.winv[kk];
     }
     __syncthreads();
     const int c = w0 / MB_CHUNK + cs;
@@ -224,19 +235,19 @@ __global__ void __launch_bounds__(MB_BLOCK)
 #pragma unroll
         for (int q = 0; q < MB_DIVG; ++q) {
           const int k = c * MB_CHUNK + hg * MB_DIVG + q;
-          const int kl = k - w0;
+          const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
           dx[q] = 0.0;
           dy[q] = 0.0;
           dz[q] = 0.0;
           num[q] = 0.0;
           den[q] = 1.0;
           if (k < N && k != i) {
-            dx[q] = sx[3 * kl + 0] - xi;
-            dy[q] = sx[3 * kl + 1] - yi;
-            dz[q] = sx[3 * kl + 2] - zi;
+            dx[q] = sx[k3 + 0] - xi;
+            dy[q] = sx[k3 + 1] - yi;
+            dz[q] = sx[k3 + 2] - zi;
             const double r2 = dot3(dx[q], dy[q], dz[q], dx[q], dy[q], dz[q]);
-            const double lok = slo[kl], wvk = swv[kl];
-            kick_nd(r2, sc.eps2, sm[kl], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, num[q], den[q]);
+            const double lok = slo[k1], wvk = swv[k1];
+            kick_nd(r2, sc.eps2, sm[k1], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, num[q], den[q]);
           }
         }
         batched_div<MB_DIVG>(num, den, w);
@@ -420,8 +431,9 @@ __global__ void __launch_bounds__(MB_BLOCK)
   const int WB = sh.CS * MB_CHUNK;
   double *qpart = lds;
   double *Qs = lds + 3 * sh.nch;
-  double *sx = Qs + 3;                 // [WB][3] partner positions after jump 1
-  double *src2 = sx + 3 * WB, *srad = src2 + WB;
+  double *sx = Qs + 3;                 // partner positions after jump 1, lds3 layout
+  double *src2 = sx + 3 * WB + sh.CS;  // per-body scalars, lds1 layout
+  double *srad = src2 + WB + sh.CS;
   wg_momentum(VA, bp.m, N, sh.nch, M, qpart, Qs);
   const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
   const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
@@ -446,11 +458,11 @@ __global__ void __launch_bounds__(MB_BLOCK)
     const int nw = min(WB, N - w0);
     for (int k = threadIdx.x; k < nw; k += blockDim.x) {
       const int kk = w0 + k;
-      sx[3 * k + 0] = fma(sc.hh, q0, XA[3 * kk + 0]);
-      sx[3 * k + 1] = fma(sc.hh, q1, XA[3 * kk + 1]);
-      sx[3 * k + 2] = fma(sc.hh, q2, XA[3 * kk + 2]);
-      src2[k] = bp.rc2[kk];
-      srad[k] = bp.rad[kk];
+      sx[lds3(k) + 0] = fma(sc.hh, q0, XA[3 * kk + 0]);
+      sx[lds3(k) + 1] = fma(sc.hh, q1, XA[3 * kk + 1]);
+      sx[lds3(k) + 2] = fma(sc.hh, q2, XA[3 * kk + 2]);
+      src2[lds1(k)] = bp.rc2[kk];
+      srad[lds1(k)] = bp.rad[kk];
     }
     __syncthreads();
     const int c = w0 / MB_CHUNK + cs;
@@ -459,14 +471,14 @@ __global__ void __launch_bounds__(MB_BLOCK)
     for (int q = 0; q < MB_CHUNK; ++q) {
       const int k = c * MB_CHUNK + q;
       if (k <= i || k >= N) continue;
-      const int kl = k - w0;
-      const double dx = sx[3 * kl + 0] - xi;
-      const double dy = sx[3 * kl + 1] - yi;
-      const double dz = sx[3 * kl + 2] - zi;
+      const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
+      const double dx = sx[k3 + 0] - xi;
+      const double dy = sx[k3 + 1] - yi;
+      const double dz = sx[k3 + 2] - zi;
       const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double sr = radi + srad[kl];
+      const double sr = radi + srad[k1];
       if (r2 < sr * sr) ++ov;
-      const double rc2k = src2[kl];
+      const double rc2k = src2[k1];
       if (r2 < (rc2i > rc2k ? rc2i : rc2k)) {
         const unsigned int slot = atomicAdd(ctl + 0, 1u);
         if (slot < (unsigned int)MB_EMAX) {
@@ -725,12 +737,17 @@ __global__ void __launch_bounds__(64)
   if (st && lane == 0) atomicOr(status + s, st);
 }
 
+// Padded staged window (lds3 / lds1 layouts): 3 WB + CS doubles of positions,
+// WB + CS per scalar array.  At the 2048-body limit (nch 128, TI 4, CS 64) the
+// kick kernel takes 63,512 bytes, within 64 KiB.
 size_t close_lds(const MBShape &sh) {
-  return ((size_t)sh.nch * sh.TI * 3 + (size_t)sh.nch * 3 + 3 + (size_t)6 * sh.CS * MB_CHUNK) * sizeof(double);
+  const size_t WB = (size_t)sh.CS * MB_CHUNK;
+  return ((size_t)sh.nch * sh.TI * 3 + 3 + (3 * WB + sh.CS) + 3 * (WB + sh.CS)) * sizeof(double);
 }
 
 size_t open_lds(const MBShape &sh) {
-  return ((size_t)sh.nch * 3 + 3 + (size_t)5 * sh.CS * MB_CHUNK) * sizeof(double);
+  const size_t WB = (size_t)sh.CS * MB_CHUNK;
+  return ((size_t)sh.nch * 3 + 3 + (3 * WB + sh.CS) + 2 * (WB + sh.CS)) * sizeof(double);
 }
 
 size_t drift_lds(const MBShape &) { return ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int); }

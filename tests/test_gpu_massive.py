@@ -44,6 +44,19 @@ def test_embryos_1000_bitwise_20_steps():
     assert rstats["massive_groups"] > 0
 
 
+def test_embryos_at_the_body_limit_bitwise():
+    """2048 bodies, the most one system takes: the largest staged windows and
+    chunk-sum buffers of the pair-check and kick kernels."""
+    st = W.embryos(2048)
+    rad = st.meta["radius"]
+    ref, rst, rstats = O.run(st, 5, 0.002, bs_tol=1e-11, radii=rad)
+    out, status, stats, _ = _gpu(st, 5, 0.002, radii=rad, bs_tol=1e-11)
+    _same(out, ref)
+    assert np.array_equal(status, rst)
+    for k in ("massive_groups", "massive_group_bodies", "overlaps"):
+        assert stats[k] == rstats[k], k
+
+
 @pytest.mark.parametrize("n,steps", [(64, 300), (200, 150)])
 def test_embryo_ring_bitwise(n, steps):
     st = embryo_ring(n)
