@@ -112,6 +112,12 @@ __device__ __forceinline__ void fold3(const double *p, int n, int stride, double
 __device__ __forceinline__ int lds3(int k) { return 3 * k + (k >> 4); }  // position x of window body k
 __device__ __forceinline__ int lds1(int k) { return k + (k >> 4); }      // scalar of window body k
 
+// Bodies a thread stages per window (windows of at most MB_MAXCS chunks).
+// The first window's data is loaded into registers before the momentum sum,
+// so that both sets of loads are in flight together; the jump is applied
+// when the values go to LDS, once the momentum is known.
+constexpr int MB_PF = MB_MAXCS * MB_CHUNK / MB_BLOCK;
+
 // Q = (sum_j m_j v_j)/M by the whole workgroup: chunk sums (from zero, index
 // order) in part[], then added in chunk order by one thread.
 __device__ void wg_momentum(const double *__restrict__ V, const double *__restrict__ m, int N,
@@ -183,46 +189,74 @@ __global__ void __launch_bounds__(MB_BLOCK)
   double *sm = sx + 3 * WB + sh.CS;                  // per-body scalars, lds1 layout
   double *slo = sm + WB + sh.CS, *swv = slo + WB + sh.CS;
   const double *X = INIT ? XA : XB;
+  const double *V = INIT ? VA : VB;
+  const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
+  const int i = (int)blockIdx.x * sh.TI + ti;
+  const bool act = i < N;
+  // this lane's body and the first window, loaded ahead of the momentum sum
+  double xi = 0.0, yi = 0.0, zi = 0.0, loi = 0.0, wvi = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  if (act) {
+    xi = X[3 * i + 0];
+    yi = X[3 * i + 1];
+    zi = X[3 * i + 2];
+    loi = bp.lo[i];
+    wvi = bp.winv[i];
+    if (cs == 0) {
+      vx = V[3 * i + 0];
+      vy = V[3 * i + 1];
+      vz = V[3 * i + 2];
+    }
+  }
+  double pf[MB_PF][6];
+#pragma unroll
+  for (int u = 0; u < MB_PF; ++u) {
+    const int k = (int)threadIdx.x + u * MB_BLOCK;
+    if (k < WB && k < N) {
+      pf[u][0] = X[3 * k + 0];
+      pf[u][1] = X[3 * k + 1];
+      pf[u][2] = X[3 * k + 2];
+      pf[u][3] = bp.m[k];
+      pf[u][4] = bp.lo[k];
+      pf[u][5] = bp.winv[k];
+    }
+  }
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (!INIT) {
     wg_momentum(VB, bp.m, N, sh.nch, M, qpart, Qs);
     q0 = Qs[0];
     q1 = Qs[1];
     q2 = Qs[2];
+    xi = fma(sc.hh, q0, xi);
+    yi = fma(sc.hh, q1, yi);
+    zi = fma(sc.hh, q2, zi);
   }
-  const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
-  const int i = (int)blockIdx.x * sh.TI + ti;
-  const bool act = i < N;
-  double xi = 0.0, yi = 0.0, zi = 0.0, loi = 0.0, wvi = 0.0;
-  if (act) {
-    xi = X[3 * i + 0];
-    yi = X[3 * i + 1];
-    zi = X[3 * i + 2];
+  auto stage = [&](int k, double xk, double yk, double zk, double mk, double lok, double wvk) {
     if (!INIT) {
-      xi = fma(sc.hh, q0, xi);
-      yi = fma(sc.hh, q1, yi);
-      zi = fma(sc.hh, q2, zi);
+      xk = fma(sc.hh, q0, xk);
+      yk = fma(sc.hh, q1, yk);
+      zk = fma(sc.hh, q2, zk);
     }
-    loi = bp.lo[i];
-    wvi = bp.winv[i];
-  }
+    sx[lds3(k) + 0] = xk;
+    sx[lds3(k) + 1] = yk;
+    sx[lds3(k) + 2] = zk;
+    sm[lds1(k)] = mk;
+    slo[lds1(k)] = lok;
+    swv[lds1(k)] = wvk;
+  };
   for (int w0 = 0; w0 < N; w0 += WB) {
     __syncthreads();
     const int nw = min(WB, N - w0);
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
-      const int kk = w0 + k;
-      double xk = X[3 * kk + 0], yk = X[3 * kk + 1], zk = X[3 * kk + 2];
-      if (!INIT) {
-        xk = fma(sc.hh, q0, xk);
-        yk = fma(sc.hh, q1, yk);
-        zk = fma(sc.hh, q2, zk);
+    if (w0 == 0) {
+#pragma unroll
+      for (int u = 0; u < MB_PF; ++u) {
+        const int k = (int)threadIdx.x + u * MB_BLOCK;
+        if (k < nw) stage(k, pf[u][0], pf[u][1], pf[u][2], pf[u][3], pf[u][4], pf[u][5]);
       }
-      sx[lds3(k) + 0] = xk;
-      sx[lds3(k) + 1] = yk;
-      sx[lds3(k) + 2] = zk;
-      sm[lds1(k)] = bp.m[kk];
-      slo[lds1(k)] = bp.lo[kk];
-      swv[lds1(k)] = bp.winv[kk];
+    } else {
+      for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+        const int kk = w0 + k;
+        stage(k, X[3 * kk + 0], X[3 * kk + 1], X[3 * kk + 2], bp.m[kk], bp.lo[kk], bp.winv[kk]);
+      }
     }
     __syncthreads();
     const int c = w0 / MB_CHUNK + cs;
@@ -268,8 +302,6 @@ __global__ void __launch_bounds__(MB_BLOCK)
   if (cs == 0 && act) {
     double ax, ay, az;
     fold3(part + 3 * ti, sh.nch, 3 * sh.TI, ax, ay, az);
-    const double *V = INIT ? VA : VB;
-    double vx = V[3 * i + 0], vy = V[3 * i + 1], vz = V[3 * i + 2];
     vx = fma(sc.hh, ax, vx);
     vy = fma(sc.hh, ay, vy);
     vz = fma(sc.hh, az, vz);
@@ -434,35 +466,64 @@ __global__ void __launch_bounds__(MB_BLOCK)
   double *sx = Qs + 3;                 // partner positions after jump 1, lds3 layout
   double *src2 = sx + 3 * WB + sh.CS;  // per-body scalars, lds1 layout
   double *srad = src2 + WB + sh.CS;
-  wg_momentum(VA, bp.m, N, sh.nch, M, qpart, Qs);
-  const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
   const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
   const int i = (int)blockIdx.x * sh.TI + ti;
   const bool act = i < N;
   unsigned int ov = 0;
+  // this lane's body and the first window, loaded ahead of the momentum sum
   double xi = 0.0, yi = 0.0, zi = 0.0, rc2i = 0.0, radi = 0.0;
   if (act) {
-    xi = fma(sc.hh, q0, XA[3 * i + 0]);
-    yi = fma(sc.hh, q1, XA[3 * i + 1]);
-    zi = fma(sc.hh, q2, XA[3 * i + 2]);
+    xi = XA[3 * i + 0];
+    yi = XA[3 * i + 1];
+    zi = XA[3 * i + 2];
+    rc2i = bp.rc2[i];
+    radi = bp.rad[i];
+  }
+  double pf[MB_PF][5];
+#pragma unroll
+  for (int u = 0; u < MB_PF; ++u) {
+    const int k = (int)threadIdx.x + u * MB_BLOCK;
+    if (k < WB && k < N) {
+      pf[u][0] = XA[3 * k + 0];
+      pf[u][1] = XA[3 * k + 1];
+      pf[u][2] = XA[3 * k + 2];
+      pf[u][3] = bp.rc2[k];
+      pf[u][4] = bp.rad[k];
+    }
+  }
+  wg_momentum(VA, bp.m, N, sh.nch, M, qpart, Qs);
+  const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
+  if (act) {
+    xi = fma(sc.hh, q0, xi);
+    yi = fma(sc.hh, q1, yi);
+    zi = fma(sc.hh, q2, zi);
     if (cs == 0) {
       Y[3 * i + 0] = xi;
       Y[3 * i + 1] = yi;
       Y[3 * i + 2] = zi;
     }
-    rc2i = bp.rc2[i];
-    radi = bp.rad[i];
   }
+  auto stage = [&](int k, double xk, double yk, double zk, double rc2k, double radk) {
+    sx[lds3(k) + 0] = fma(sc.hh, q0, xk);
+    sx[lds3(k) + 1] = fma(sc.hh, q1, yk);
+    sx[lds3(k) + 2] = fma(sc.hh, q2, zk);
+    src2[lds1(k)] = rc2k;
+    srad[lds1(k)] = radk;
+  };
   for (int w0 = 0; w0 < N; w0 += WB) {
     __syncthreads();
     const int nw = min(WB, N - w0);
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
-      const int kk = w0 + k;
-      sx[lds3(k) + 0] = fma(sc.hh, q0, XA[3 * kk + 0]);
-      sx[lds3(k) + 1] = fma(sc.hh, q1, XA[3 * kk + 1]);
-      sx[lds3(k) + 2] = fma(sc.hh, q2, XA[3 * kk + 2]);
-      src2[lds1(k)] = bp.rc2[kk];
-      srad[lds1(k)] = bp.rad[kk];
+    if (w0 == 0) {
+#pragma unroll
+      for (int u = 0; u < MB_PF; ++u) {
+        const int k = (int)threadIdx.x + u * MB_BLOCK;
+        if (k < nw) stage(k, pf[u][0], pf[u][1], pf[u][2], pf[u][3], pf[u][4]);
+      }
+    } else {
+      for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+        const int kk = w0 + k;
+        stage(k, XA[3 * kk + 0], XA[3 * kk + 1], XA[3 * kk + 2], bp.rc2[kk], bp.rad[kk]);
+      }
     }
     __syncthreads();
     const int c = w0 / MB_CHUNK + cs;
