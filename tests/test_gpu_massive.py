@@ -88,6 +88,27 @@ def test_every_group_lane_layout_bitwise(n, span, steps, mean_size):
         assert stats[k] == rstats[k], k
 
 
+def test_many_body_systems_side_by_side():
+    """Three 40-body systems in one call (one workgroup row per system): a
+    ring with encounter groups, a cluster with groups and overlaps, and a
+    quiet ring.  Every system ends bit for bit where the oracle puts it, with
+    its own status bits, and the middle one as if it had run alone."""
+    parts = [embryo_ring(40, seed=1), embryo_cluster(40, span=0.25, seed=2),
+             embryo_ring(40, seed=3, a_lo=1.5, a_hi=1.53)]
+    rad = np.concatenate([p.meta["radius"] for p in parts])
+    st = mint.State(np.zeros(4, np.int64), np.concatenate([p.m for p in parts]),
+                    np.concatenate([p.mx for p in parts]), np.concatenate([p.mv for p in parts]),
+                    np.zeros((0, 3)), np.zeros((0, 3)))
+    ref, rst, rstats = O.run(st, 100, 0.002, bs_tol=1e-11, radii=rad)
+    out, status, stats, _ = _gpu(st, 100, 0.002, radii=rad, bs_tol=1e-11)
+    _same(out, ref)
+    assert np.array_equal(status, rst) and len(set(rst.tolist())) == 3
+    for k in ("massive_groups", "massive_group_bodies", "overlaps"):
+        assert stats[k] == rstats[k], k
+    alone, _, _ = O.run(parts[1], 100, 0.002, bs_tol=1e-11, radii=parts[1].meta["radius"])
+    assert np.array_equal(out.mx[1], alone["mx"][0]) and np.array_equal(out.mv[1], alone["mv"][0])
+
+
 def test_lane_engine_and_many_body_engine_agree_on_8_bodies():
     st = embryo_cluster(8, seed=0)
     ref, rst, rstats = O.run(st, 400, 0.002, bs_tol=1e-11)
