@@ -349,9 +349,9 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 
 static_assert(2 * MB_EMAX >= 128, "the first 64 pairs are read ahead of the pair count");
 
-// e01 holds the lane's entries lane and lane + 64 of the pair list, loaded
-// by the caller alongside the pair count (the first 64 pairs).
-__device__ int build_groups(int N, int ne, const int *__restrict__ edge, const int *e01,
+// e0, e1: the lane's entries lane and lane + 64 of the pair list, loaded by
+// the caller alongside the pair count (the first 64 pairs).
+__device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0, int e1,
                             unsigned int *bm, int *pre, int *fl, int *L, int *ce, int *cnt, int *gst,
                             int *gm, bool &big) {
   const int lane = threadIdx.x;  // one wavefront
@@ -359,7 +359,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, const i
   bm[lane] = 0u;
   __syncthreads();
   for (int p = lane; p < n2; p += 64) {
-    const int b = p < 128 ? e01[p >> 6] : edge[p];
+    const int b = p < 64 ? e0 : (p < 128 ? e1 : edge[p]);
     ce[p] = b;
     atomicOr(&bm[b >> 5], 1u << (b & 31));
   }
@@ -753,7 +753,7 @@ __global__ void __launch_bounds__(64)
   const int *edge = edge_all + (size_t)s * MB_EMAX * 2;
   // the pair count and the first 64 pairs (2 MB_EMAX entries are there to
   // read; those past the count are not used) in flight together
-  const int e01[2] = {edge[threadIdx.x], edge[threadIdx.x + 64]};
+  const int e0 = edge[threadIdx.x], e1 = edge[threadIdx.x + 64];
   const int ne = (int)ctl_all[4 * s];
   const bool lead = (int)blockIdx.x == nK && threadIdx.x == 0;
   if (ne == 0) return;
@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(64)
   int *pre = ilds + 64, *fl = pre + 64, *L = fl + 2 * MB_EMAX, *ce = L + 2 * MB_EMAX;
   int *cnt = ce + 2 * MB_EMAX, *gst = cnt + 2 * MB_EMAX, *gm = gst + 2 * MB_EMAX + 1;
   bool big = false;
-  const int G = build_groups(N, ne, edge, e01, bm, pre, fl, L, ce, cnt, gst, gm, big);
+  const int G = build_groups(N, ne, edge, e0, e1, bm, pre, fl, L, ce, cnt, gst, gm, big);
   if (lead) {
     atomicOr(status + s, big ? (ST_MASSIVE_ENC | ST_LIMIT) : ST_MASSIVE_ENC);
     if (stats) {
