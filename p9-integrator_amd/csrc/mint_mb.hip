@@ -13,8 +13,9 @@
 //                    meanwhile each group workgroup turns the pair list into
 //                    encounter groups (connected components, listed by
 //                    smallest member) in its LDS and drifts its share of them
-//                    by Bulirsch-Stoer (one wavefront per group, one member
-//                    per lane, positions exchanged through LDS).
+//                    by Bulirsch-Stoer (one wavefront per group: row groups
+//                    of lanes, one member per lane, run the Gragg rows side
+//                    by side; positions exchanged by wavefront shuffles).
 //   mb_close_kernel  momentum after the drift, jump 2, O(N^2) kick
 //                    accelerations and the closing kick (fused with the next
 //                    step's opening kick, which sees the same positions).
