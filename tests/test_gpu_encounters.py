@@ -52,6 +52,24 @@ def test_permuting_tps_permutes_results():
     assert np.array_equal(a.tx[perm], b.tx) and np.array_equal(a.tv[perm], b.tv)
 
 
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_BATCHED])
+def test_tp_trajectory_does_not_depend_on_the_other_tps(form):
+    """A TP ends bit for bit in the same place whichever other TPs share its
+    system (and its encounter queue, whose order the other TPs change): every
+    third TP of an encounter cloud run alone against the whole cloud."""
+    st = encounter_system(n_tp=400, seed=9)
+    idx = np.arange(0, st.n_tp, 3)
+    sub = st.copy()
+    sub.tx, sub.tv = st.tx[idx], st.tv[idx]
+    sub.tp_offset = np.array([0, len(idx)], np.int64)
+    kw = dict(chunk_steps=16, block_size=64, tps_per_lane=1) if form == mint.FORM_BATCHED else {}
+    a, _, sa = run_gpu(st, 40, 0.5, form, mint.GUESS_SERIES, **kw)
+    b, _, sb = run_gpu(sub, 40, 0.5, form, mint.GUESS_SERIES, **kw)
+    assert sb["bs_steps"] > 100 and sa["bs_steps"] > sb["bs_steps"]
+    assert np.array_equal(a.tx[idx], b.tx) and np.array_equal(a.tv[idx], b.tv)
+    assert np.array_equal(a.mx, b.mx) and np.array_equal(a.mv, b.mv)
+
+
 def test_run_twice_is_deterministic_and_launch_shape_invariant():
     st = encounter_system(n_tp=300, seed=3)
     a, _, _ = run_gpu(st, 30, 0.5, mint.FORM_BATCHED, mint.GUESS_SERIES, chunk_steps=8, block_size=64,
