@@ -50,6 +50,32 @@ def planet_cloud(nm, n_tp=96, seed=0):
                       meta=dict(workload="planet_cloud", nm=nm))
 
 
+def unbound_cloud(n=256, seed=17):
+    """The giants of sweep system 0 and n TPs at 20-200 AU in four sets of
+    speed, as multiples of the escape speed: hyperbolic (1.5-20), near the
+    parabolic limit (1 +- 1e-9 .. 1e-3), around escape (0.9-1.1) and slow
+    (0.05-0.3); every fourth TP moves nearly radially."""
+    st = W.p9_sweep(systems=[0], tp_scale=1 / 64).copy()
+    rng = np.random.default_rng([20261203, 10, seed])
+    q = n // 4
+    r = rng.uniform(20.0, 200.0, n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = rng.normal(size=(n, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    radial = np.arange(n) % 4 == 3
+    w[radial] = u[radial] + 1e-3 * w[radial]
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    sign = np.where(rng.uniform(size=q) < 0.5, -1.0, 1.0)
+    f = np.concatenate([rng.uniform(1.5, 20.0, q), 1.0 + sign * 10.0 ** rng.uniform(-9, -3, q),
+                        rng.uniform(0.9, 1.1, q), rng.uniform(0.05, 0.3, n - 3 * q)])
+    v = f * np.sqrt(2.0 / r)  # central mass 1, G = 1
+    st.tx = u * r[:, None]
+    st.tv = w * v[:, None]
+    st.tp_offset = np.array([0, n], np.int64)
+    return st
+
+
 def run_gpu(st, nsteps, dt, form, guess, stats=1, **kw):
     with mint.Mint(dt=dt, form=form, kepler_guess=guess, collect_stats=stats, **kw) as g:
         g.upload(st)

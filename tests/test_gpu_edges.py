@@ -1,14 +1,15 @@
 """Edge cases of the batched form against the oracle, bit for bit: step
 counts that are not a multiple of the window (and shorter than one), one-step
 windows, a single TP, and systems without TPs between systems with them,
-with close encounters carried across window boundaries."""
+with close encounters carried across window boundaries; non-finite TPs; TPs
+on unbound and near-parabolic orbits (plain form too)."""
 import numpy as np
 import pytest
 
 import mint
 import mint_workloads as W
 import oracle as O
-from gpu_cases import encounter_system, run_gpu
+from gpu_cases import encounter_system, run_gpu, unbound_cloud
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +75,20 @@ def test_non_finite_tps_are_flagged_and_isolated(form):
     assert not np.all(np.isfinite(out_b.tx[i_nan])) and not np.all(np.isfinite(out_b.tv[i_inf]))
     _, rst, _ = O.run(bad, 40, 0.5, guess=1)
     assert np.array_equal(rst & mint.ST_NAN, stat_b & mint.ST_NAN)
+
+
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_BATCHED])
+def test_unbound_and_near_parabolic_tps_match_the_oracle(form):
+    """TPs on hyperbolic orbits, within 1e-9 of the parabolic limit, around
+    escape and slow and nearly radial: the universal-variable drift's
+    negative-alpha Stumpff branches and large-|z| quartering over 100 steps,
+    bit for bit against the oracle."""
+    st = unbound_cloud()
+    print(f"unbound cloud form={form}", flush=True)
+    kw = dict(chunk_steps=32, block_size=64, tps_per_lane=1) if form == mint.FORM_BATCHED else {}
+    out, status, s = run_gpu(st, 100, 0.5, form, mint.GUESS_SERIES, **kw)
+    ref, rst, rs = O.run(st, 100, 0.5, guess=1)
+    assert np.all(np.isfinite(ref["tx"])) and np.abs(ref["tx"]).max() > 200.0
+    assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
+    assert np.array_equal(out.mx, ref["mx"]) and np.all(status == rst)
+    assert s["newton_iters"] == rs["newton_iters"] > rs["kepler_solves"]
