@@ -858,6 +858,25 @@ void orc_accel_massive(int32_t NM, const double *m, const double *rc, const doub
   }
 }
 
+// Kick acceleration of a test particle at x from NM massive bodies at X
+// (body-major xyz), changeover radii rc[NM] (body_interaction_kick for a
+// massless body, src/sim.cuh:327-382).
+int orc_accel_tp(int32_t NM, const double *m, const double *rc, const double *X, const double *x, double eps2,
+                 double *a) {
+  if (NM < 1 || NM > MAXB) return -1;
+  Sys s;
+  s.nm = NM;
+  s.m.assign(m, m + NM);
+  for (int j = 0; j < NM; ++j) s.rc_tp.push_back(make_rc(rc[j]));
+  std::vector<V3> P(NM);
+  for (int j = 0; j < NM; ++j) P[j] = {X[3 * j], X[3 * j + 1], X[3 * j + 2]};
+  const V3 q = accel_tp({x[0], x[1], x[2]}, P, s, eps2);
+  a[0] = q.x;
+  a[1] = q.y;
+  a[2] = q.z;
+  return 0;
+}
+
 // Encounter groups of NM massive bodies at positions x: gid[j] = group number
 // (groups numbered by smallest member) or -1.  Returns the number of groups;
 // *overlaps counts pairs closer than the sum of their radii (rad may be null).

@@ -53,6 +53,8 @@ def lib():
         L.orc_accel_massive.argtypes = [C.c_int32, _DP, _DP, _DP, C.c_double, _DP]
         L.orc_encounter_groups.argtypes = [C.c_int32, _DP, _DP, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.orc_encounter_groups.restype = C.c_int
+        L.orc_accel_tp.argtypes = [C.c_int32, _DP, _DP, _DP, _DP, C.c_double, _DP]
+        L.orc_accel_tp.restype = C.c_int
         _L = L
     return _L
 
@@ -145,6 +147,18 @@ def accel_massive(m, rc, x, eps2=1e-8):
     x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
     a = np.empty_like(x)
     lib().orc_accel_massive(len(m), _d(m), _d(rc), _d(x), eps2, _d(a))
+    return a
+
+
+def accel_tp(m, rc, X, x, eps2=1e-8):
+    """Kick acceleration of one test particle at x from the massive bodies at X."""
+    m = np.ascontiguousarray(m, np.float64).ravel()
+    rc = np.ascontiguousarray(rc, np.float64).ravel()
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    a = np.empty(3)
+    if lib().orc_accel_tp(len(m), _d(m), _d(rc), _d(X), _d(x), eps2, _d(a)) != 0:
+        raise ValueError("oracle: 1..8 massive bodies")
     return a
 
 
