@@ -82,7 +82,12 @@ typedef struct mint_config {
   int32_t use_graph;     /* batched: replay chunk launches from a captured hipGraph; default 1 */
   int32_t device;        /* HIP device ordinal; default 0 (the -d flag, src/simutils.cuh:170-173) */
   int32_t collect_stats; /* bit 0: count work into mint_stats; bit 1: per-kernel HIP-event
-                            timing into mint_timing (launches without the graph); default 0 */
+                            timing into mint_timing (launches without the graph, one stream);
+                            bit 2: batched form, time each main-kernel launch of the normal
+                            schedule (graph and streams unchanged) into ms_tp_main /
+                            n_tp_main, from the device wall clock stamped by every
+                            wavefront at entry and exit (last exit - first entry);
+                            default 0 */
   int32_t many_body;     /* 1: batches without TPs use the many-body engine at any n_massive
                             (same results; default 0 = only above 8 massive bodies) */
 } mint_config;
@@ -159,6 +164,14 @@ int mint_upload(void *ctx, const mint_state *st);
  * per-batch launch loop src/mercurius.cu:179-198 and the step loop body
  * src/sim.cuh:679-742).  Returns after the device work has finished. */
 int mint_step(void *ctx, int64_t nsteps);
+
+/* Build what a later mint_step(ctx, nsteps) replays (the batched form's
+ * captured launch graph for an nsteps call at the current launch shape and
+ * constants) without running anything, so that the host-side capture and
+ * instantiation are not part of that call.  No-op for the other forms.  Host
+ * work only (no counterpart in the reference, which launches its kernel
+ * directly, src/mercurius.cu:184-196). */
+int mint_prepare(void *ctx, int64_t nsteps);
 
 /* Copy the state back, converting velocities to heliocentric form (replaces
  * the teardown src/sim.cuh:744-765 and the D2H copies src/mercurius.cu:203-243).
@@ -257,6 +270,16 @@ int mint_nn_loss_grad(void *ctx, int32_t which, int64_t n, const float *feat, co
 int mint_nn_nparams(int32_t which);
 /* Pre-bucket a TP when the scorer's largest pair logit exceeds t (default 0). */
 int mint_set_score_threshold(void *ctx, float t);
+/* Scorer pairs far from a massive body are left out of the network's work
+ * when mode = 1 (the default): the library takes the smallest cutoff c of a
+ * ladder (1.5 .. 8, in units of 8 r_c) for which an interval bound of the
+ * network's logit over every pair with distance and straight-line closest
+ * approach over the window both at least 8 c r_c lies below the threshold,
+ * so that no pick can change; mode = 0 scores every pair; mode = -1 leaves
+ * the mode as it is.  *bound receives that bound, *cutoff that c (+inf: no
+ * safe c, nothing is left out), *active whether pairs are left out now (any
+ * output may be NULL).  No counterpart in the reference, which has no scorer. */
+int mint_scorer_skip(void *ctx, int32_t mode, float *bound, float *cutoff, int32_t *active);
 int mint_nn_get_weights(void *ctx, int32_t which, float *w);
 int mint_nn_set_weights(void *ctx, int32_t which, const float *w);
 

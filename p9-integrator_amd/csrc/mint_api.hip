@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -45,15 +46,24 @@ int Ctx::hip(hipError_t e, const char *what) {
   return MINT_EHIP;
 }
 
+void Ctx::drop_graphs() {
+  for (auto &g : graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  graphs.clear();
+  if (mb_graph) hipGraphExecDestroy(mb_graph);
+  mb_graph = nullptr;
+  mb_graph_steps = 0;
+}
+
 void Ctx::free_device() {
   for (void *p : allocs) hipFree(p);
   allocs.clear();
-  for (auto &g : graphs) {
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  }
-  graphs.clear();
+  drop_graphs();
   d_prm = d_mx = d_mv = d_xs0 = d_rec = nullptr;
   d_xs0b = d_recb = d_xs0c = d_recc = nullptr;
+  for (auto &p : d_snap) p = nullptr;
+  d_kstamp = nullptr;
+  kstamp_cap = 0;
   d_pc = nullptr;
   d_carry = nullptr;
   d_slist = nullptr;
@@ -69,9 +79,6 @@ void Ctx::free_device() {
   d_status = nullptr;
   d_stats = nullptr;
   d_nnw_kep = d_nnw_sc = nullptr;
-  if (mb_graph) hipGraphExecDestroy(mb_graph);
-  mb_graph = nullptr;
-  mb_graph_steps = 0;
   d_bp = d_mxb = d_mvb = d_my = nullptr;
   d_mflag = d_medge = nullptr;
   d_mctl = nullptr;
@@ -96,9 +103,7 @@ int Ctx::ensure_rec(int C) {
     hipFree(d_rec);
     allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)d_rec));
     d_rec = nullptr;
-    for (auto &g : graphs)
-      if (g.exec) hipGraphExecDestroy(g.exec);
-    graphs.clear();
+    drop_graphs();
   }
   if (d_pc) {
     hipFree(d_pc);
@@ -119,6 +124,8 @@ int Ctx::ensure_rec2(int C) {
   int rc;
   if (!d_q2 && (rc = dalloc((void **)&d_q2, (size_t)std::max<long long>(Ntp, 1) * sizeof(QEntry), "BS queue (odd windows)")))
     return rc;
+  for (auto &p : d_snap)
+    if (!p && (rc = dalloc((void **)&p, (size_t)S * NM * 6 * sizeof(double), "window-start snapshot"))) return rc;
   if (C <= rec2_steps_alloc) return MINT_OK;
   for (double **p : {&d_recb, &d_xs0b, &d_recc, &d_xs0c}) {
     if (*p) {
@@ -127,9 +134,7 @@ int Ctx::ensure_rec2(int C) {
       *p = nullptr;
     }
   }
-  for (auto &g : graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  graphs.clear();
+  drop_graphs();
   const size_t bytes = (size_t)S * (size_t)C * (size_t)rec_size(NM) * sizeof(double);
   const size_t xb = (size_t)S * NM * 3 * sizeof(double);
   if ((rc = dalloc((void **)&d_recb, bytes, "ephemeris (second buffer)"))) return rc;
@@ -138,6 +143,24 @@ int Ctx::ensure_rec2(int C) {
   if ((rc = dalloc((void **)&d_xs0c, xb, "chunk start positions (third buffer)"))) return rc;
   rec2_steps_alloc = C;
   return MINT_OK;
+}
+
+GraphSig Ctx::graph_sig() const {
+  const StepConst sc = step_const();
+  GraphSig g;
+  g.h = sc.h;
+  g.mu = sc.mu;
+  g.eps2 = sc.eps2;
+  g.tol = sc.tol;
+  g.gate_ir0 = sc.gate_ir0;
+  g.M = cfg.central_mass;
+  g.thresh = score_thresh;
+  g.guess = cfg.kepler_guess;
+  g.scorer = cfg.use_scorer;
+  g.stats = cfg.collect_stats;
+  g.skip = scorer_skip_on() ? 1 : 0;
+  g.cut = sc_cut;
+  return g;
 }
 
 StepConst Ctx::step_const() const {
@@ -194,9 +217,7 @@ int Ctx::ensure_tiles(int block, int P) {
     if (rc) return rc;
   }
   tiles_key = key;
-  for (auto &g : graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  graphs.clear();
+  drop_graphs();
   return MINT_OK;
 }
 
@@ -380,9 +401,12 @@ int mint_hill_radii(int64_t n, double M, double n_hill, const double *m, const d
   return MINT_OK;
 }
 
-int mint_upload(void *ctx, const mint_state *st) {
-  Ctx *c = (Ctx *)ctx;
-  if (!c) return MINT_EINVAL;
+namespace {
+
+// mint_upload proper.  Every argument is checked before the context is
+// touched; a failure after that point leaves the context with nothing
+// uploaded (the wrapper clears the flag), never with mixed old and new state.
+int upload_impl(Ctx *c, const mint_state *st) {
   if (!st) return c->fail(MINT_EINVAL, "mint_upload: null state");
   const int S = st->n_systems, NM = st->n_massive;
   if (S <= 0 || NM < 1 || NM > MB_MAXN)
@@ -403,12 +427,17 @@ int mint_upload(void *ctx, const mint_state *st) {
   for (int i = 0; i < S * NM; ++i)
     if (!(st->m[i] > 1e-12))
       return c->fail(MINT_EINVAL, "mint_upload: massive masses must exceed 1e-12 (src/mercurius.cu:9)");
+  if (st->rc)
+    for (int i = 0; i < S * NM; ++i)
+      if (!(st->rc[i] > 0.0) || !std::isfinite(st->rc[i]))
+        return c->fail(MINT_EINVAL, "mint_upload: rc must be positive and finite");
   hipSetDevice(c->cfg.device);
   hipStreamSynchronize(c->stream);
   // same shape as the resident batch: keep the buffers (and captured graphs)
   const bool reuse = c->uploaded && c->S == S && c->NM == NM && c->Ntp == N &&
                      std::equal(c->tp_offset.begin(), c->tp_offset.end(), st->tp_offset);
   if (!reuse) c->free_device();
+  c->uploaded = false;
   c->S = S;
   c->NM = NM;
   c->Ntp = N;
@@ -419,10 +448,7 @@ int mint_upload(void *ctx, const mint_state *st) {
   c->m.assign(st->m, st->m + (size_t)S * NM);
   c->rc.resize((size_t)S * NM);
   if (st->rc) {
-    for (int i = 0; i < S * NM; ++i) {
-      if (!(st->rc[i] > 0.0)) return c->fail(MINT_EINVAL, "mint_upload: rc must be positive");
-      c->rc[i] = st->rc[i];
-    }
+    for (int i = 0; i < S * NM; ++i) c->rc[i] = st->rc[i];
   } else {
     mint_hill_radii((int64_t)S * NM, c->cfg.central_mass, c->cfg.n_hill, st->m, st->mx, st->mv,
                     c->rc.data());
@@ -528,6 +554,21 @@ int mint_upload(void *ctx, const mint_state *st) {
   return MINT_OK;
 }
 
+}  // namespace
+
+int mint_upload(void *ctx, const mint_state *st) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  int rc;
+  try {
+    rc = upload_impl(c, st);
+  } catch (const std::bad_alloc &) {
+    rc = c->fail(MINT_ENOMEM, "mint_upload: host allocation failed");
+  }
+  if (rc) c->uploaded = false;
+  return rc;
+}
+
 int mint_download(void *ctx, mint_state *st) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
@@ -617,25 +658,35 @@ int mint_save_checkpoint(void *ctx, const char *path, int64_t step_index) {
   return ok ? MINT_OK : c->fail(MINT_EIO, "mint_save_checkpoint: write failed");
 }
 
-int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
-  Ctx *c = (Ctx *)ctx;
-  if (!c) return MINT_EINVAL;
+namespace {
+
+// mint_load_checkpoint proper.  The header is checked against the file's
+// size before anything is allocated, and the context's step parameters are
+// only replaced once the whole file has been read; if the upload then fails,
+// the previous parameters are restored and nothing is uploaded.
+int load_checkpoint_impl(Ctx *c, const char *path, int64_t *step_index) {
   if (!path) return c->fail(MINT_EINVAL, "mint_load_checkpoint: null path");
   FILE *f = std::fopen(path, "rb");
   if (!f) return c->fail(MINT_EIO, std::string("mint_load_checkpoint: cannot open ") + path);
+  long fsize = -1;
+  if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+  std::rewind(f);
   char magic[8];
   int32_t hdr_i[3];
   int64_t hdr_l[2];
   double hdr_d[5];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, MINT_CKPT_MAGIC, 8) == 0 &&
+  bool ok = fsize > 0 && std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, MINT_CKPT_MAGIC, 8) == 0 &&
             std::fread(hdr_i, sizeof(hdr_i), 1, f) == 1 && std::fread(hdr_l, sizeof(hdr_l), 1, f) == 1 &&
             std::fread(hdr_d, sizeof(hdr_d), 1, f) == 1;
   const int S = ok ? hdr_i[0] : 0, NM = ok ? hdr_i[1] : 0;
   const int64_t N = ok ? hdr_l[0] : 0;
   ok = ok && S > 0 && NM >= 1 && NM <= MB_MAXN && N >= 0 && N < 0x7fffffff && hdr_i[2] == 2;
-  if (!ok) {
+  // the body: tp_offset[S+1], then m, rc, radii [S*NM], mx, mv [S*NM*3], six TP arrays [N]
+  const long long head = 8 + (long long)sizeof(hdr_i) + (long long)sizeof(hdr_l) + (long long)sizeof(hdr_d);
+  const long long expect = ok ? head + 8LL * ((long long)S + 1) + 8LL * 9 * (long long)S * NM + 8LL * 6 * N : -1;
+  if (!ok || expect != (long long)fsize) {
     std::fclose(f);
-    return c->fail(MINT_EIO, "mint_load_checkpoint: not a mint checkpoint or truncated header");
+    return c->fail(MINT_EIO, "mint_load_checkpoint: not a mint checkpoint, or its size disagrees with its header");
   }
   std::vector<int64_t> off((size_t)S + 1);
   std::vector<double> m((size_t)S * NM), rcv((size_t)S * NM), rad((size_t)S * NM), mx((size_t)S * NM * 3),
@@ -651,27 +702,46 @@ int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
   for (int k = 0; k < 6 && ok; ++k) ok = std::fread(t[k].data(), sizeof(double), (size_t)N, f) == (size_t)N;
   std::fclose(f);
   if (!ok || off[0] != 0 || off[S] != N) return c->fail(MINT_EIO, "mint_load_checkpoint: truncated or inconsistent file");
+  const mint_config saved = c->cfg;
   c->cfg.dt = hdr_d[0];
   c->cfg.central_mass = hdr_d[1];
   c->cfg.n_hill = hdr_d[2];
   c->cfg.softening2 = hdr_d[3];
   c->cfg.bs_tol = hdr_d[4];
-  // upload with the heliocentric conversion undone: pass DH velocities shifted
-  // back is lossy, so upload the geometry and overwrite the state raw
+  // upload the geometry with zero massive velocities (no DH shift), then
+  // overwrite the velocities with the saved internal (DH) ones, bit for bit
   std::vector<double> zeros_v(mv.size(), 0.0);
   mint_state st{S, NM, off.data(), m.data(), mx.data(), zeros_v.data(), t[0].data(), t[1].data(), t[2].data(),
                 t[3].data(), t[4].data(), t[5].data(), rcv.data()};
-  int rc = mint_upload(ctx, &st);
-  if (rc) return rc;
-  hipSetDevice(c->cfg.device);
-  if ((rc = c->hip(hipMemcpy(c->d_mv, mv.data(), mv.size() * sizeof(double), hipMemcpyHostToDevice), "H2D"))) return rc;
-  for (int k = 3; k < 6; ++k)
-    if (N > 0 && (rc = c->hip(hipMemcpy(c->d_tp[k], t[k].data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice),
-                              "H2D")))
-      return rc;
-  if ((rc = mint_set_radii(ctx, rad.data()))) return rc;
+  int rc = mint_upload(c, &st);
+  if (!rc) {
+    hipSetDevice(c->cfg.device);
+    rc = c->hip(hipMemcpy(c->d_mv, mv.data(), mv.size() * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    for (int k = 3; k < 6 && !rc; ++k)
+      if (N > 0)
+        rc = c->hip(hipMemcpy(c->d_tp[k], t[k].data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    if (!rc) rc = mint_set_radii(c, rad.data());
+  }
+  if (rc) {
+    c->cfg = saved;
+    c->uploaded = false;
+    return rc;
+  }
   if (step_index) *step_index = hdr_l[1];
   return MINT_OK;
+}
+
+}  // namespace
+
+int mint_load_checkpoint(void *ctx, const char *path, int64_t *step_index) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  try {
+    return load_checkpoint_impl(c, path, step_index);
+  } catch (const std::bad_alloc &) {
+    c->uploaded = false;
+    return c->fail(MINT_ENOMEM, "mint_load_checkpoint: host allocation failed");
+  }
 }
 
 int mint_set_radii(void *ctx, const double *radius) {
@@ -795,9 +865,7 @@ int mint_set_mode(void *ctx, int32_t form, int32_t guess, int32_t use_scorer) {
   if (!c) return MINT_EINVAL;
   if (form < 0 || form > 2 || guess < 0 || guess > 2) return c->fail(MINT_EINVAL, "mint_set_mode: bad form or guess");
   if (c->cfg.form != form || c->cfg.kepler_guess != guess || c->cfg.use_scorer != use_scorer) {
-    for (auto &g : c->graphs)
-      if (g.exec) hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    c->drop_graphs();
   }
   c->cfg.form = form;
   c->cfg.kepler_guess = guess;

@@ -10,11 +10,27 @@
 
 namespace mint {
 
-// A captured launch window (ephemeris + queue reset + main + encounter) for
-// one window length C; replayed once per window.
+// What a captured graph holds by value besides device pointers (the step
+// constants, the central mass, the scorer threshold, the Kepler guess, the
+// scorer and counter switches): a cached graph is replayed only while all of
+// these are unchanged, so a change of dt or tolerance (mint_load_checkpoint)
+// or of the threshold can never replay old constants.
+struct GraphSig {
+  double h = 0, mu = 0, eps2 = 0, tol = 0, gate_ir0 = 0, M = 0;
+  float thresh = 0.f, cut = 0.f;
+  int guess = -1, scorer = -1, stats = -1, skip = -1;
+  bool same(const GraphSig &o) const {
+    return h == o.h && mu == o.mu && eps2 == o.eps2 && tol == o.tol && gate_ir0 == o.gate_ir0 && M == o.M &&
+           thresh == o.thresh && cut == o.cut && guess == o.guess && scorer == o.scorer && stats == o.stats && skip == o.skip;
+  }
+};
+
+// The captured launch windows of one mint_step(nsteps) call of the batched form.
 struct GraphEntry {
   long long nsteps = 0;
   int block = 0, P = 0, C = 0;  // launch shape the graph was captured with
+  int nwin = 0;                 // launch windows in the graph
+  GraphSig sig;
   hipGraphExec_t exec = nullptr;
 };
 
@@ -29,6 +45,15 @@ struct Ctx {
   // per-window events of the pipeline: ephemeris done, window done, queue
   // work forked from / joined back into the context stream
   std::vector<hipEvent_t> ev_eph, ev_tp, ev_fork, ev_join;
+  // collect_stats bit 2: every main-kernel launch of a batched call stamps
+  // the wall clock at each wavefront's entry and exit, window w into
+  // d_kstamp[w * 2 * kstamp_wpl ...]; kwin_pending windows of the call just
+  // enqueued await reading
+  long long *d_kstamp = nullptr;
+  size_t kstamp_cap = 0;  // long longs
+  int kstamp_wpl = 0;     // wavefronts per main-kernel launch
+  int kwin_pending = 0;
+  double wall_khz = 0.0;  // wall-clock rate
 
   // batch shape
   int S = 0, NM = 0;
@@ -44,6 +69,12 @@ struct Ctx {
   // ephemeris buffers 2 and 3 of the batched form (three rotate: a window's
   // buffer is in use until its carried tails finish in the next window)
   double *d_xs0b = nullptr, *d_recb = nullptr, *d_xs0c = nullptr, *d_recc = nullptr;
+  // massive positions and velocities at the start of launch window w, in
+  // buffer w mod 3 (positions [S][NM][3], then velocities): what the queue
+  // head of window w reads, written by the ephemeris of window w-1 (by a copy
+  // for the first window of a call), so that the queue head does not wait for
+  // its own window's ephemeris
+  double *d_snap[3] = {nullptr, nullptr, nullptr};
   double *d_pc = nullptr;          // perturber cache of the window being served
   unsigned int *d_need = nullptr;  // [2][S] per window parity: 1 queue head, 2 mid-window deferral
   int *d_slist = nullptr;          // [2][2][S] per parity: systems flagged 1, then flagged 2
@@ -67,6 +98,13 @@ struct Ctx {
   // learned parts (host copies; empty = not trained / not set)
   std::vector<float> w_kep, w_sc;
   float score_thresh = 0.0f;
+  // scorer pairs with both distance features at or above sc_cut (feature
+  // units) are left out of the network's list when mode 1: sc_cut is set with
+  // the weights and the threshold, from an interval bound of the logit over
+  // such pairs (sc_sat_bound) that lies below the threshold
+  int scorer_skip_mode = 1;
+  float sc_sat_bound = INFINITY, sc_cut = INFINITY;
+  bool scorer_skip_on() const { return scorer_skip_mode == 1 && !w_sc.empty() && sc_cut <= 8.0f; }
 
   // launch shape chosen for the batched form
   int tuned_block = 0, tuned_P = 0, tuned_C = 0;
@@ -86,6 +124,7 @@ struct Ctx {
   std::vector<hipEvent_t> tev;       // event pool of the per-kernel timing mode
   hipGraphExec_t mb_graph = nullptr;
   long long mb_graph_steps = 0;
+  GraphSig mb_sig;                  // what mb_graph was captured with
 
   int fail(int code, const std::string &msg);
   int hip(hipError_t e, const char *what);
@@ -95,6 +134,8 @@ struct Ctx {
   int ensure_rec2(int C);
   int ensure_tiles(int block, int P);
   StepConst step_const() const;
+  GraphSig graph_sig() const;
+  void drop_graphs();
   TPArrays tp_arrays() const;
 };
 
@@ -150,6 +191,7 @@ int mb_set_radii(Ctx *c);   // body parameters again after a change of rad
 int run_many(Ctx *c, int64_t nsteps);
 
 extern std::string g_create_error;
+float scorer_cutoff(const std::vector<float> &w, float t, float *bound);
 const char *form_name(int f);
 
 }  // namespace mint

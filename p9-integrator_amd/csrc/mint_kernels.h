@@ -1,7 +1,7 @@
 // mint_kernels.h — gfx950 kernels of the Mercurius step.
 //
 //  ephemeris_kernel   massive subsystem of each system for C steps; one lane
-//                     per system; writes the per-step ephemeris records that
+//                     per massive body; writes the per-step ephemeris records that
 //                     every TP phase reads (replaces the cluster-wide reads of
 //                     block 0's shared memory, src/sim.cuh:15-26,682-741)
 //  tp_step_kernel     one step for one TP per lane (plain and flat forms)
@@ -179,6 +179,141 @@ __device__ __forceinline__ int tp_kepler(double &x, double &y, double &z, double
 }
 
 // ---------------------------------------------------------------------------
+// Massive subsystem, lane per body: a group of EPH_L(NM) lanes per system
+// (the body count rounded up to a power of two), lane j holding body j, so a
+// step's five Kepler drifts, kicks and pair checks run side by side instead of
+// in one lane.  Each lane forms exactly what iteration j of the body loops of
+// accel_massive, momentum_q and encounter_graph forms, in the same order, with
+// the other bodies' positions and velocities taken from their lanes by
+// shuffles; the records are therefore the same bits.  Lanes beyond NM shadow
+// the last body (nothing of theirs is written).
+template <int NM>
+constexpr int EPH_L = NM <= 1 ? 1 : (NM <= 2 ? 2 : (NM <= 4 ? 4 : 8));
+
+// Per-lane copies of body j's parameters, read once per launch (held in
+// registers across the step loop): partner masses and the pair changeover
+// fields max/min over (j, k), and the squared sums of physical radii.
+template <int NM>
+struct LaneParams {
+  double m[NM], prclo[NM], prcwinv[NM], prc2[NM], sr2[NM];
+};
+
+template <int NM>
+__device__ __forceinline__ void lane_params(const double *__restrict__ prm, int j, LaneParams<NM> &p) {
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    p.m[k] = prm[P_M(NM) + k];
+    p.prclo[k] = prm[P_PRCLO(NM) + j * NM + k];
+    p.prcwinv[k] = prm[P_PRCWINV(NM) + j * NM + k];
+    p.prc2[k] = prm[P_PRC2(NM) + j * NM + k];
+    const double sr = prm[P_RAD(NM) + j] + prm[P_RAD(NM) + k];
+    p.sr2[k] = sr * sr;
+  }
+}
+
+// Kick acceleration of body j (this lane) from the bodies of its lane group
+// (lanes base .. base+NM-1): iteration j of accel_massive.
+template <int NM>
+__device__ __forceinline__ void accel_lane(const double *x, int base, int j, const LaneParams<NM> &p,
+                                           double eps2, double *a) {
+  double dx[NM], dy[NM], dz[NM], num[NM], den[NM], w[NM];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    const double xk = __shfl(x[0], base + k, 64);
+    const double yk = __shfl(x[1], base + k, 64);
+    const double zk = __shfl(x[2], base + k, 64);
+    dx[k] = xk - x[0];
+    dy[k] = yk - x[1];
+    dz[k] = zk - x[2];
+    const double r2 = dot3(dx[k], dy[k], dz[k], dx[k], dy[k], dz[k]);
+    kick_nd(r2, eps2, p.m[k], p.prclo[k], p.prcwinv[k], num[k], den[k]);
+    if (k == j) {  // the body's own slot is inert
+      dx[k] = 0.0;
+      dy[k] = 0.0;
+      dz[k] = 0.0;
+      num[k] = 0.0;
+      den[k] = 1.0;
+    }
+  }
+  batched_div<NM>(num, den, w);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    ax = fma(w[k], dx[k], ax);
+    ay = fma(w[k], dy[k], ay);
+    az = fma(w[k], dz[k], az);
+  }
+  a[0] = ax;
+  a[1] = ay;
+  a[2] = az;
+}
+
+// Q = (sum_k m_k v_k) / M over the lane group, the order of momentum_q.
+template <int NM>
+__device__ __forceinline__ void momentum_lane(const double *v, int base, const LaneParams<NM> &p, double M,
+                                              double *Q) {
+  double px = 0.0, py = 0.0, pz = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    const double m = p.m[k];
+    px = fma(m, __shfl(v[0], base + k, 64), px);
+    py = fma(m, __shfl(v[1], base + k, 64), py);
+    pz = fma(m, __shfl(v[2], base + k, 64), pz);
+  }
+  Q[0] = px / M;
+  Q[1] = py / M;
+  Q[2] = pz / M;
+}
+
+// Encounter groups of one system, from its bodies' states S6[NM][6] (x, v)
+// and adjacency rows adj[NM] (bit k of adj[j]: pair (j, k) inside its
+// changeover radius): each connected component, members in index order, is
+// drifted by one BS as one system, in place; *ingrp receives the bodies in a
+// group.  The component walk and the BS are those of the one-lane form.  Out
+// of line: taken only when some pair of the system is that close.
+template <int NM>
+__device__ __noinline__ uint32_t massive_groups(double *S6, const uint32_t *adj, const double *__restrict__ prm,
+                                                StepConst sc, unsigned long long *stats, uint32_t *ingrp) {
+  uint32_t st = ST_MASSIVE_ENC, in = 0u;
+  unsigned int ng = 0, nb = 0, nr = 0;
+  for (int r = 0; r < NM; ++r) {
+    if (adj[r] == 0u || ((in >> r) & 1u)) continue;
+    uint32_t comp = 1u << r, front = comp;
+    while (front) {
+      uint32_t nxt = 0u;
+      for (int j = 0; j < NM; ++j)
+        if ((front >> j) & 1u) nxt |= adj[j];
+      front = nxt & ~comp;
+      comp |= nxt;
+    }
+    in |= comp;
+    int idx[NM], g = 0;
+    double sv[6 * NM];
+    for (int j = 0; j < NM; ++j) {
+      if (!((comp >> j) & 1u)) continue;
+      for (int c = 0; c < 6; ++c) sv[6 * g + c] = S6[6 * j + c];
+      idx[g++] = j;
+    }
+    st |= bs_drift_group<NM>(sv, g, idx, sc.h, prm, sc.mu, sc.eps2, sc.tol, nr);
+    for (int q = 0; q < g; ++q)
+      for (int c = 0; c < 6; ++c) S6[6 * idx[q] + c] = sv[6 * q + c];
+    ++ng;
+    nb += (unsigned int)g;
+  }
+  if (stats) {
+    atomicAdd(stats + S_MGROUPS, (unsigned long long)ng);
+    atomicAdd(stats + S_MGBODIES, (unsigned long long)nb);
+  }
+  *ingrp = in;
+  return st;
+}
+
+// The massive subsystem of systems [s_begin, s_end) for C steps: the
+// per-step ephemeris records every TP phase reads (replaces the cluster-wide
+// reads of block 0's shared memory, src/sim.cuh:15-26,682-741), then the
+// state at the end of the C steps, into (mx, mv) and, when nx is not null,
+// also into (nx, nv): the next launch window's start, which the queue head of
+// that window reads.  One-wave blocks of 64 / EPH_L systems.
 template <int NM>
 __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, int C,
                                                        const double *__restrict__ prm_all,
@@ -186,120 +321,141 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
                                                        double *__restrict__ mv,
                                                        double *__restrict__ xs0,
                                                        double *__restrict__ rec,
+                                                       double *__restrict__ nx,
+                                                       double *__restrict__ nv,
                                                        unsigned int *__restrict__ status,
                                                        unsigned long long *__restrict__ stats,
                                                        StepConst sc, double M) {
+  constexpr int L = EPH_L<NM>;
+  constexpr int SPW = 64 / L;  // systems per wave
   constexpr int REC = rec_size(NM);
-  const int s = s_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (s >= s_end) return;
+  __shared__ double sh_state[SPW][NM][6];
+  __shared__ uint32_t sh_adj[SPW][NM];
+  __shared__ uint32_t sh_grp[SPW];
+  const int lane = (int)threadIdx.x;
+  const int gs = lane / L, j = lane % L, base = gs * L;
+  const int s_raw = s_begin + (int)blockIdx.x * SPW + gs;
+  const bool sys_ok = s_raw < s_end;
+  const int s = sys_ok ? s_raw : s_end - 1;  // a group past the end shadows the last system
+  const bool own = sys_ok && j < NM;
+  const int jb = j < NM ? j : NM - 1;
   const double *prm = prm_all + (size_t)s * prm_size(NM);
-  double X[3 * NM], V[3 * NM], A[3 * NM], Q[3];
+  LaneParams<NM> lp;
+  lane_params<NM>(prm, jb, lp);
+  double x[3], v[3], a[3], Q[3];
 #pragma unroll
-  for (int i = 0; i < 3 * NM; ++i) {
-    X[i] = mx[(size_t)s * 3 * NM + i];
-    V[i] = mv[(size_t)s * 3 * NM + i];
-    xs0[(size_t)s * 3 * NM + i] = X[i];
+  for (int c = 0; c < 3; ++c) {
+    x[c] = mx[((size_t)s * NM + jb) * 3 + c];
+    v[c] = mv[((size_t)s * NM + jb) * 3 + c];
+    if (own) xs0[((size_t)s * NM + jb) * 3 + c] = x[c];
   }
-  accel_massive<NM>(X, prm, sc.eps2, A);
+  accel_lane<NM>(x, base, jb, lp, sc.eps2, a);
   uint32_t st = 0;
   for (int k = 0; k < C; ++k) {
     double *r = rec + ((size_t)s * C + k) * REC;
 #pragma unroll
-    for (int i = 0; i < 3 * NM; ++i) V[i] = fma(sc.hh, A[i], V[i]);
-    momentum_q<NM>(V, prm, M, Q);
+    for (int c = 0; c < 3; ++c) v[c] = fma(sc.hh, a[c], v[c]);
+    momentum_lane<NM>(v, base, lp, M, Q);
 #pragma unroll
-    for (int j = 0; j < NM; ++j)
+    for (int c = 0; c < 3; ++c) x[c] = fma(sc.hh, Q[c], x[c]);
+    if (own) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) X[3 * j + c] = fma(sc.hh, Q[c], X[3 * j + c]);
-    r[R_Q1(NM) + 0] = Q[0];
-    r[R_Q1(NM) + 1] = Q[1];
-    r[R_Q1(NM) + 2] = Q[2];
-#pragma unroll
-    for (int i = 0; i < 3 * NM; ++i) {
-      r[R_Y(NM) + i] = X[i];
-      r[R_W(NM) + i] = V[i];
+      for (int c = 0; c < 3; ++c) {
+        r[R_Y(NM) + 3 * jb + c] = x[c];
+        r[R_W(NM) + 3 * jb + c] = v[c];
+      }
+      if (jb == 0) {
+        r[R_Q1(NM) + 0] = Q[0];
+        r[R_Q1(NM) + 1] = Q[1];
+        r[R_Q1(NM) + 2] = Q[2];
+      }
     }
-    // encounter groups: connected components of the changeover graph, each
-    // drifted by BS as one system; every other body drifts on its Kepler orbit
-    uint32_t adj[NM];
+    // pair checks of encounter_graph: the separation of a pair is formed as
+    // (higher index) - (lower index) in both of its lanes; overlaps are
+    // counted by the lower index's lane
+    uint32_t adj = 0u;
     unsigned int ov = 0;
-    uint32_t ingrp = 0u;
-    if (encounter_graph<NM>(X, prm, adj, ov)) {
-      st |= ST_MASSIVE_ENC;
-      unsigned int ng = 0, nb = 0, nr = 0;
 #pragma unroll
-      for (int r = 0; r < NM; ++r) {
-        if (adj[r] == 0u || ((ingrp >> r) & 1u)) continue;
-        uint32_t comp = 1u << r, front = comp;
-        while (front) {
-          uint32_t nxt = 0u;
-#pragma unroll
-          for (int j = 0; j < NM; ++j)
-            if ((front >> j) & 1u) nxt |= adj[j];
-          front = nxt & ~comp;
-          comp |= nxt;
-        }
-        ingrp |= comp;
-        int idx[NM], g = 0;
-        double sv[6 * NM];
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-          if (!((comp >> j) & 1u)) continue;
-          for (int c = 0; c < 3; ++c) {
-            sv[6 * g + c] = X[3 * j + c];
-            sv[6 * g + 3 + c] = V[3 * j + c];
-          }
-          idx[g++] = j;
-        }
-        st |= bs_drift_group<NM>(sv, g, idx, sc.h, prm, sc.mu, sc.eps2, sc.tol, nr);
-        for (int a = 0; a < g; ++a) {
-          const int j = idx[a];
-          for (int c = 0; c < 3; ++c) {
-            X[3 * j + c] = sv[6 * a + c];
-            V[3 * j + c] = sv[6 * a + 3 + c];
-          }
-        }
-        ++ng;
-        nb += (unsigned int)g;
-      }
-      if (stats) {
-        atomicAdd(stats + S_MGROUPS, (unsigned long long)ng);
-        atomicAdd(stats + S_MGBODIES, (unsigned long long)nb);
-      }
+    for (int q = 0; q < NM; ++q) {
+      const double xq = __shfl(x[0], base + q, 64);
+      const double yq = __shfl(x[1], base + q, 64);
+      const double zq = __shfl(x[2], base + q, 64);
+      const bool hi = q > jb;
+      const double dx = hi ? xq - x[0] : x[0] - xq;
+      const double dy = hi ? yq - x[1] : x[1] - yq;
+      const double dz = hi ? zq - x[2] : x[2] - zq;
+      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (hi && r2 < lp.sr2[q]) ++ov;
+      if (q != jb && r2 < lp.prc2[q]) adj |= 1u << q;
+    }
+    if (!own) {
+      adj = 0u;
+      ov = 0;
     }
     if (ov) {
       st |= ST_OVERLAP;
       if (stats) atomicAdd(stats + S_OVERLAP, (unsigned long long)ov);
     }
+    bool ingrp = false;
+    if (__builtin_amdgcn_ballot_w64(adj != 0u) != 0ull) {  // some system of this wave has a close pair
+      const unsigned long long gb = __builtin_amdgcn_ballot_w64(adj != 0u);
+      const bool sys_enc = ((gb >> base) & ((L == 64 ? ~0ull : (1ull << L) - 1ull))) != 0ull;
+      if (own) {
 #pragma unroll
-    for (int j = 0; j < NM; ++j) {
-      if ((ingrp >> j) & 1u) continue;
+        for (int c = 0; c < 3; ++c) {
+          sh_state[gs][jb][c] = x[c];
+          sh_state[gs][jb][3 + c] = v[c];
+        }
+        sh_adj[gs][jb] = adj;
+      }
+      __syncthreads();
+      if (own && jb == 0) {
+        uint32_t in = 0u;
+        if (sys_enc) st |= massive_groups<NM>(&sh_state[gs][0][0], &sh_adj[gs][0], prm, sc, stats, &in);
+        sh_grp[gs] = in;
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          x[c] = sh_state[gs][jb][c];
+          v[c] = sh_state[gs][jb][3 + c];
+        }
+        ingrp = ((sh_grp[gs] >> jb) & 1u) != 0u;
+      }
+      __syncthreads();
+    }
+    if (!ingrp) {
       bool ok;
-      kepler_drift_series(X[3 * j], X[3 * j + 1], X[3 * j + 2], V[3 * j], V[3 * j + 1],
-                          V[3 * j + 2], sc.mu, sc.h, ok);
+      kepler_drift_series(x[0], x[1], x[2], v[0], v[1], v[2], sc.mu, sc.h, ok);
       if (!ok) st |= ST_NEWTON;
     }
-    momentum_q<NM>(V, prm, M, Q);
+    momentum_lane<NM>(v, base, lp, M, Q);
 #pragma unroll
-    for (int j = 0; j < NM; ++j)
+    for (int c = 0; c < 3; ++c) x[c] = fma(sc.hh, Q[c], x[c]);
+    if (own && jb == 0) {
+      r[R_Q2(NM) + 0] = Q[0];
+      r[R_Q2(NM) + 1] = Q[1];
+      r[R_Q2(NM) + 2] = Q[2];
+    }
+    accel_lane<NM>(x, base, jb, lp, sc.eps2, a);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) X[3 * j + c] = fma(sc.hh, Q[c], X[3 * j + c]);
-    r[R_Q2(NM) + 0] = Q[0];
-    r[R_Q2(NM) + 1] = Q[1];
-    r[R_Q2(NM) + 2] = Q[2];
-    accel_massive<NM>(X, prm, sc.eps2, A);
-#pragma unroll
-    for (int i = 0; i < 3 * NM; ++i) {
-      V[i] = fma(sc.hh, A[i], V[i]);
-      r[R_XE(NM) + i] = X[i];
+    for (int c = 0; c < 3; ++c) {
+      v[c] = fma(sc.hh, a[c], v[c]);
+      if (own) r[R_XE(NM) + 3 * jb + c] = x[c];
     }
   }
+  if (!own) return;
   bool fin = true;
 #pragma unroll
-  for (int i = 0; i < 3 * NM; ++i) {
-    mx[(size_t)s * 3 * NM + i] = X[i];
-    mv[(size_t)s * 3 * NM + i] = V[i];
-    fin = fin && isfinite(X[i]) && isfinite(V[i]);
+  for (int c = 0; c < 3; ++c) {
+    mx[((size_t)s * NM + jb) * 3 + c] = x[c];
+    mv[((size_t)s * NM + jb) * 3 + c] = v[c];
+    if (nx) {
+      nx[((size_t)s * NM + jb) * 3 + c] = x[c];
+      nv[((size_t)s * NM + jb) * 3 + c] = v[c];
+    }
+    fin = fin && isfinite(x[c]) && isfinite(v[c]);
   }
   if (!fin) st |= ST_NAN;
   if (st) atomicOr(&status[s], st);
@@ -417,18 +573,21 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
 
 // ---------------------------------------------------------------------------
 // Scorer features of one (TP, massive body) pair at the start of a launch
-// window of duration T: distance, approach rate and closest approach in units
-// of r_c, and Hill-zone size ratios.  Differences are taken in FP64
+// window of duration T (the body's position and velocity at the window
+// start, before its first half kick): distance, approach rate and closest approach in units
+// of r_c, and Hill-zone size ratios.  Every feature is clamped to a fixed
+// range (f0, f3 in [0, 8], f1 in [-4, 4], f2, f4 in [0, 4], f5 in [0, 2]),
+// which is what lets the host bound the network's output over a feature box.  Differences are taken in FP64
 // (positions are hundreds of AU); the rest is FP32 with the hardware square
 // root, reciprocal square root and reciprocal (about one ulp each): the scorer
 // only ranks pairs, and the exact distance check decides every step.
 __device__ __forceinline__ void scorer_features(double x, double y, double z, double vx, double vy,
                                                 double vz, double X, double Y, double Z, double VX,
-                                                double VY, double VZ, double rc, double T,
+                                                double VY, double VZ, float rcf, double T,
                                                 float *f) {
   const float dx = (float)(X - x), dy = (float)(Y - y), dz = (float)(Z - z);
   const float ux = (float)(VX - vx), uy = (float)(VY - vy), uz = (float)(VZ - vz);
-  const float Tf = (float)T, rcf = (float)rc, irc = __builtin_amdgcn_rcpf(rcf);
+  const float Tf = (float)T, irc = __builtin_amdgcn_rcpf(rcf);
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
   const float du = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
@@ -445,38 +604,31 @@ __device__ __forceinline__ void scorer_features(double x, double y, double z, do
   f[1] = fmaxf(fminf(((du * __builtin_amdgcn_rcpf(r)) * irc) * Tf, 16.0f), -16.0f) * 0.25f;
   f[2] = fminf((__builtin_amdgcn_sqrtf(u2) * Tf) * irc, 16.0f) * 0.25f;
   f[3] = fminf(rmin * irc, 64.0f) * 0.125f;
-  f[4] = (rcf * iR) * 4.0f;
+  f[4] = fminf(rcf * iR, 1.0f) * 4.0f;
   f[5] = fminf(rt * iR, 8.0f) * 0.25f;
 }
 
-// Largest scorer logit over the NM pairs of one TP, f[j] the features of
-// pair j.  Hidden unit outermost: each unit's 8 weights are read once (scalar
-// loads) and applied to every pair, so the weights never pile up in scalar
-// registers.  Per pair the sums run in the order of scorer_fwd.
-template <int NM>
-__device__ __forceinline__ float scorer_best(const float (*f)[SC_IN], const float *__restrict__ w) {
-  float out[NM];
-#pragma unroll
-  for (int j = 0; j < NM; ++j) out[j] = w[SC_B2];
-#pragma unroll 2
-  for (int i = 0; i < SC_H; ++i) {
-    const float b1 = w[SC_B1 + i], w2 = w[SC_W2 + i];
-    float w1[SC_IN];
-#pragma unroll
-    for (int k = 0; k < SC_IN; ++k) w1[k] = w[SC_W1 + i * SC_IN + k];
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-      float a = b1;
-#pragma unroll
-      for (int k = 0; k < SC_IN; ++k) a = fmaf(w1[k], f[j][k], a);
-      out[j] = fmaf(w2, nn_tanh(a), out[j]);
-    }
-  }
-  float best = -1e30f;
-#pragma unroll
-  for (int j = 0; j < NM; ++j) best = fmaxf(best, out[j]);
-  return best;
+// r_c of pair body j in FP32, from r_c^2 (the scorer's unit of length).
+__device__ __forceinline__ float scorer_rc(double rc2) { return __builtin_amdgcn_sqrtf((float)rc2); }
+
+// Whether the scorer features of a pair are certain to have both distance
+// features at or above `cut` (f0, f3 >= cut: distance and straight-line
+// closest approach over the window at least 8 cut r_c).  Sufficient, without
+// a square root: r^2 >= 2 (A^2 + (|u| T)^2) implies r >= A + |u| T, and
+// |d + u t| >= |d| - |u| T over the window; A = 8 cut r_c and every term carry
+// a margin of 1e-3 over the FP32 roundings of the features.  Conservative: a
+// pair it does not rule out is checked on its formed features.
+__device__ __forceinline__ bool scorer_far(double x, double y, double z, double vx, double vy, double vz,
+                                           double X, double Y, double Z, double VX, double VY, double VZ,
+                                           float rcf, float Tf, float cut) {
+  const float dx = (float)(X - x), dy = (float)(Y - y), dz = (float)(Z - z);
+  const float ux = (float)(VX - vx), uy = (float)(VY - vy), uz = (float)(VZ - vz);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+  const float A = (8.0f * 1.001f) * cut * rcf, uT2 = (1.001f * 1.001f) * (Tf * Tf);
+  return r2 * 0.999f >= 2.0f * fmaf(u2, uT2, A * A);
 }
+
 
 // One TP per lane: registers are capped for four waves per SIMD (128 VGPRs;
 // the few that spill are off the step loop's common path), which measured
@@ -490,13 +642,20 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     const unsigned char *__restrict__ pre, QEntry *__restrict__ q,
     unsigned int *__restrict__ qcount, unsigned int *__restrict__ need, int *__restrict__ slist,
     unsigned char *__restrict__ carry, unsigned int *__restrict__ status,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, long long *__restrict__ kstamp) {
   constexpr int REC = rec_size(NM);
+  // kstamp (launch timing): each wavefront's first lane stamps the
+  // constant-rate wall clock at entry and exit into its own pair of slots
+  const long long wslot = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (kstamp && (threadIdx.x & 63) == 0) kstamp[2 * wslot] = wall_clock64();
   // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so the tiles of one system share an L2.
   const int b = blockIdx.x;
   const int t_idx = (b & 7) * tiles_per_xcd + (b >> 3);
-  if (t_idx >= ntiles) return;
+  if (t_idx >= ntiles) {
+    if (kstamp && (threadIdx.x & 63) == 0) kstamp[2 * wslot + 1] = wall_clock64();
+    return;
+  }
   const Tile tile = tiles[t_idx];
   const int s = tile.sys;
   const double *prm = prm_all + (size_t)s * prm_size(NM);
@@ -620,6 +779,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     wave_add_uniform(stats + S_DEFER, n_start - n_end);
     wave_add_uniform(stats + S_IDLE, wave_sum(slots) * (unsigned)C - n_step);
   }
+  if (kstamp && (threadIdx.x & 63) == 0) kstamp[2 * wslot + 1] = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -631,41 +791,117 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 // step inside the queue, so results do not change.  Carry flags are cleared
 // here; a carried TP's state is not read (its previous window may still be
 // finishing), the queue entry reads it when served.
+//
+// The network runs over a packed list: each lane forms the features of its
+// NM pairs, the block gathers the pairs to be scored into LDS, and its 256
+// lanes score the list, one pair each per pass.  Pairs with both distance
+// features at or above `cut` (distance and straight-line closest approach
+// over the window at least 8 cut r_c) are left out of the list: the host
+// sets cut (+inf: none left out) only where an interval bound of the
+// network's logit over every such pair lies below the threshold, so no pick
+// changes.  With the trained weights cut is 1.5 (12 r_c); on the 4096-system
+// sweep about a quarter of a pair per TP is then scored.
+constexpr int PB_BLOCK = 256;
+
 template <int NM>
-__global__ void __launch_bounds__(256) tp_prebucket_kernel(
+__global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     long long n, const int *__restrict__ tp_sys, TPArrays tp, const double *__restrict__ prm_all,
-    const double *__restrict__ xs0, const double *__restrict__ rec, int C, StepConst sc,
-    const float *__restrict__ scw, float thresh, unsigned char *__restrict__ carry,
+    const double *__restrict__ wx, const double *__restrict__ wv, int C, StepConst sc,
+    const float *__restrict__ scw, float thresh, float cut, unsigned char *__restrict__ carry,
     unsigned char *__restrict__ pre, QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
     unsigned int *__restrict__ need, int *__restrict__ slist, unsigned long long *__restrict__ stats) {
-  constexpr int REC = rec_size(NM);
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float sh_f[PB_BLOCK * NM][SC_IN];  // packed pair features
+  __shared__ unsigned short sh_item[PB_BLOCK * NM];  // owner lane * 8 + body
+  __shared__ float sh_logit[PB_BLOCK][NM];
+  __shared__ unsigned int sh_n;
+  const int t = (int)threadIdx.x;
+  const long long i = (long long)blockIdx.x * PB_BLOCK + t;
   unsigned int n_pre = 0;
+  bool go = false;
+  int s = -1;
   if (i < n) {
-    bool go = carry[i] != 0;
+    go = carry[i] != 0;
     if (go) carry[i] = 0;
-    int s = -1;
-    if (!go && scw != nullptr) {
+  }
+  if (scw != nullptr) {  // uniform: every lane takes part in the block's phases
+    if (t == 0) sh_n = 0u;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) sh_logit[t][j] = -INFINITY;
+    float f[NM][SC_IN];
+    unsigned int want = 0u;
+    if (i < n && !go) {
       s = tp_sys[i];
       const double *prm = prm_all + (size_t)s * prm_size(NM);
-      const double *xsys = xs0 + (size_t)s * 3 * NM;
-      const double *r0 = rec + (size_t)s * C * REC;
+      const double *bx = wx + (size_t)s * 3 * NM, *bv = wv + (size_t)s * 3 * NM;
       const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
       const double T = sc.h * (double)C;
-      float f[NM][SC_IN];
 #pragma unroll
-      for (int j = 0; j < NM; ++j)
-        scorer_features(x, y, z, vx, vy, vz, xsys[3 * j], xsys[3 * j + 1], xsys[3 * j + 2],
-                        r0[R_W(NM) + 3 * j], r0[R_W(NM) + 3 * j + 1], r0[R_W(NM) + 3 * j + 2],
-                        sqrt(prm[P_RC2(NM) + j]), T, f[j]);
-      go = scorer_best<NM>(f, scw) > thresh;
+      for (int j = 0; j < NM; ++j) {
+        const float rcf = scorer_rc(prm[P_RC2(NM) + j]);
+        if (cut <= 8.0f && scorer_far(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
+                                      bv[3 * j + 1], bv[3 * j + 2], rcf, (float)T, cut))
+          continue;
+        scorer_features(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
+                        bv[3 * j + 1], bv[3 * j + 2], rcf, T, f[j]);
+        if (!(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
+      }
+    }
+    // slots: a wavefront prefix sum of the lanes' counts, one LDS add per wavefront
+    const unsigned int cnt = (unsigned int)__builtin_popcount(want);
+    unsigned int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned int o = __shfl_up(incl, off, 64);
+      if ((t & 63) >= off) incl += o;
+    }
+    const unsigned int wtot = __shfl(incl, 63, 64);
+    __syncthreads();  // sh_n and sh_logit initialised
+    unsigned int wbase = 0u;
+    if ((t & 63) == 63) wbase = atomicAdd(&sh_n, wtot);
+    wbase = __shfl(wbase, 63, 64);
+    unsigned int slot = wbase + incl - cnt;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      if (!((want >> j) & 1u)) continue;
+#pragma unroll
+      for (int k = 0; k < SC_IN; ++k) sh_f[slot][k] = f[j][k];
+      sh_item[slot] = (unsigned short)(t * 8 + j);
+      ++slot;
+    }
+    __syncthreads();
+    const unsigned int ni = sh_n;
+    for (unsigned int it = (unsigned int)t; it < ni; it += PB_BLOCK) {
+      float g[SC_IN];
+#pragma unroll
+      for (int k = 0; k < SC_IN; ++k) g[k] = sh_f[it][k];
+      const unsigned int own = sh_item[it];
+      // scorer_fwd's sums in its order, one hidden unit's 8 weights loaded
+      // at a time (all 129 held at once would spill from scalar registers)
+      float out = scw[SC_B2];
+#pragma unroll 2
+      for (int u = 0; u < SC_H; ++u) {
+        float a = scw[SC_B1 + u];
+#pragma unroll
+        for (int k = 0; k < SC_IN; ++k) a = fmaf(scw[SC_W1 + u * SC_IN + k], g[k], a);
+        out = fmaf(scw[SC_W2 + u], nn_tanh(a), out);
+      }
+      sh_logit[own >> 3][own & 7u] = out;
+    }
+    __syncthreads();
+    if (i < n && !go) {
+      float best = -INFINITY;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) best = fmaxf(best, sh_logit[t][j]);
+      go = best > thresh;
       n_pre = go ? 1u : 0u;
     }
+  }
+  if (i < n) {
     pre[i] = go ? 1 : 0;
     if (go) {
       if (s < 0) s = tp_sys[i];
-      const unsigned int slot = atomicAdd(qcount, 1u);
-      QEntry *e = q + slot;
+      const unsigned int qs = atomicAdd(qcount, 1u);
+      QEntry *e = q + qs;
       e->tp = (int)i;
       e->sys = s;
       e->k0 = 0;
@@ -804,6 +1040,17 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
 }
 
 static __global__ void copy_u32_kernel(unsigned int *dst, const unsigned int *src) { *dst = *src; }
+
+// Window-start massive state of a call's first launch window (the ephemeris
+// writes it for every later window).
+static __global__ void copy_f64_kernel(double *__restrict__ dx, double *__restrict__ dv, const double *__restrict__ sx,
+                                       const double *__restrict__ sv, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    dx[i] = sx[i];
+    dv[i] = sv[i];
+  }
+}
 
 static __global__ void zero_u32_kernel(unsigned int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -9,7 +9,7 @@ namespace mint {
 
 struct Launchers {
   hipError_t (*ephemeris)(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
-                          double *mx, double *mv, double *xs0, double *rec,
+                          double *mx, double *mv, double *xs0, double *rec, double *nx, double *nv,
                           unsigned int *status, unsigned long long *stats, StepConst sc,
                           double M);
   hipError_t (*tp_step)(hipStream_t st, int guess, int block, long long i0, long long i1,
@@ -22,7 +22,7 @@ struct Launchers {
                         StepConst sc, const float *nnw, const unsigned char *pre,
                         QEntry *q, unsigned int *qcount, unsigned int *need, int *slist,
                         unsigned char *carry, unsigned int *status,
-                        unsigned long long *stats);
+                        unsigned long long *stats, long long *kstamp);
   hipError_t (*tp_encounter)(hipStream_t st, int guess, int grid, int block, const QEntry *q,
                              const unsigned int *qcount, const unsigned int *qbegin, int dense,
                              int grouped, TPArrays tp,
@@ -30,8 +30,8 @@ struct Launchers {
                              const double *pc, int C, StepConst sc, const float *nnw,
                              unsigned int *status, unsigned long long *stats);
   hipError_t (*tp_prebucket)(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
-                             const double *prm, const double *xs0, const double *rec, int C,
-                             StepConst sc, const float *scw, float thresh,
+                             const double *prm, const double *wx, const double *wv, int C,
+                             StepConst sc, const float *scw, float thresh, float cut,
                              unsigned char *carry, unsigned char *pre, QEntry *q,
                              unsigned int *qcount, unsigned int *need, int *slist,
                              unsigned long long *stats);

@@ -875,9 +875,15 @@ int enqueue_step(Ctx *c, int kicks, int last, KTimer *kt) {
 }
 
 int mb_graph(Ctx *c, hipGraphExec_t *out) {
-  if (c->mb_graph && c->mb_graph_steps == MB_GRAPH_STEPS) {
+  const GraphSig sig = c->graph_sig();
+  if (c->mb_graph && c->mb_graph_steps == MB_GRAPH_STEPS && c->mb_sig.same(sig)) {
     *out = c->mb_graph;
     return MINT_OK;
+  }
+  if (c->mb_graph) {  // captured with other constants (a checkpoint of another dt, say)
+    hipGraphExecDestroy(c->mb_graph);
+    c->mb_graph = nullptr;
+    c->mb_graph_steps = 0;
   }
   int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
@@ -893,8 +899,14 @@ int mb_graph(Ctx *c, hipGraphExec_t *out) {
   e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if ((rc = c->hip(e, "graph instantiate"))) return rc;
+  if ((rc = c->hip(hipGraphUpload(ex, c->stream), "graph upload")) ||
+      (rc = c->hip(hipStreamSynchronize(c->stream), "graph upload"))) {
+    hipGraphExecDestroy(ex);
+    return rc;
+  }
   c->mb_graph = ex;
   c->mb_graph_steps = MB_GRAPH_STEPS;
+  c->mb_sig = sig;
   *out = ex;
   return MINT_OK;
 }

@@ -11,12 +11,13 @@ namespace {
 constexpr int NMV = MINT_NM;
 
 hipError_t ephemeris_l(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
-                       double *mx, double *mv, double *xs0, double *rec, unsigned int *status,
-                       unsigned long long *stats, StepConst sc, double M) {
+                       double *mx, double *mv, double *xs0, double *rec, double *nx, double *nv,
+                       unsigned int *status, unsigned long long *stats, StepConst sc, double M) {
   const int n = s_end - s_begin;
   if (n <= 0) return hipSuccess;
-  ephemeris_kernel<NMV><<<(n + 63) / 64, 64, 0, st>>>(s_begin, s_end, C, prm, mx, mv, xs0, rec,
-                                                      status, stats, sc, M);
+  constexpr int spw = 64 / EPH_L<NMV>;  // systems per one-wave block
+  ephemeris_kernel<NMV><<<(n + spw - 1) / spw, 64, 0, st>>>(s_begin, s_end, C, prm, mx, mv, xs0, rec,
+                                                            nx, nv, status, stats, sc, M);
   return hipGetLastError();
 }
 
@@ -53,10 +54,10 @@ hipError_t main_pg(hipStream_t st, int block, int nblocks, const Tile *tiles, in
                    const double *rec, int C, StepConst sc, const float *nnw,
                    const unsigned char *pre, QEntry *q, unsigned int *qcount,
                    unsigned int *need, int *slist, unsigned char *carry, unsigned int *status,
-                   unsigned long long *stats) {
+                   unsigned long long *stats, long long *kstamp) {
   tp_main_kernel<NMV, P, G><<<nblocks, block, 0, st>>>(tiles, ntiles, tiles_per_xcd, tp, prm, xs0,
                                                        rec, C, sc, nnw, pre, q, qcount, need,
-                                                       slist, carry, status, stats);
+                                                       slist, carry, status, stats, kstamp);
   return hipGetLastError();
 }
 
@@ -65,12 +66,12 @@ hipError_t tp_main_l(hipStream_t st, int guess, int P, int block, int nblocks, c
                      const double *xs0, const double *rec, int C, StepConst sc, const float *nnw,
                      const unsigned char *pre, QEntry *q, unsigned int *qcount,
                      unsigned int *need, int *slist, unsigned char *carry, unsigned int *status,
-                     unsigned long long *stats) {
+                     unsigned long long *stats, long long *kstamp) {
   if (block <= 0 || block > 512 || (block & 63)) return hipErrorInvalidValue;
 #define MINT_MAIN(PP, GG)                                                                       \
   if (P == PP && guess == GG)                                                                   \
     return main_pg<PP, GG>(st, block, nblocks, tiles, ntiles, tiles_per_xcd, tp, prm, xs0, rec, \
-                           C, sc, nnw, pre, q, qcount, need, slist, carry, status, stats);
+                           C, sc, nnw, pre, q, qcount, need, slist, carry, status, stats, kstamp);
   MINT_MAIN(1, 0) MINT_MAIN(1, 1) MINT_MAIN(1, 2)
   MINT_MAIN(2, 0) MINT_MAIN(2, 1) MINT_MAIN(2, 2)
   MINT_MAIN(4, 0) MINT_MAIN(4, 1) MINT_MAIN(4, 2)
@@ -120,13 +121,13 @@ hipError_t tp_encounter_l(hipStream_t st, int guess, int grid, int block, const 
 }
 
 hipError_t tp_prebucket_l(hipStream_t st, long long n, const int *tp_sys, TPArrays tp,
-                          const double *prm, const double *xs0, const double *rec, int C,
-                          StepConst sc, const float *scw, float thresh, unsigned char *carry,
+                          const double *prm, const double *wx, const double *wv, int C,
+                          StepConst sc, const float *scw, float thresh, float cut, unsigned char *carry,
                           unsigned char *pre, QEntry *q, unsigned int *qcount, unsigned int *need,
                           int *slist, unsigned long long *stats) {
   if (n <= 0) return hipSuccess;
-  tp_prebucket_kernel<NMV><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-      n, tp_sys, tp, prm, xs0, rec, C, sc, scw, thresh, carry, pre, q, qcount, need, slist, stats);
+  tp_prebucket_kernel<NMV><<<(unsigned)((n + PB_BLOCK - 1) / PB_BLOCK), PB_BLOCK, 0, st>>>(
+      n, tp_sys, tp, prm, wx, wv, C, sc, scw, thresh, cut, carry, pre, q, qcount, need, slist, stats);
   return hipGetLastError();
 }
 

@@ -56,7 +56,8 @@ __global__ void kepler_trace_kernel(const double *__restrict__ x, const double *
 template <int NM>
 __global__ void encounter_trace_kernel(TPArrays tp, const int *__restrict__ tp_sys, long long n,
                                        long long stride, const double *__restrict__ prm_all,
-                                       const double *__restrict__ xs0, const double *__restrict__ rec,
+                                       const double *__restrict__ xs0, const double *__restrict__ vs0,
+                                       const double *__restrict__ rec,
                                        int C, StepConst sc, float *__restrict__ feat,
                                        float *__restrict__ label, unsigned int *__restrict__ counter,
                                        unsigned int cap, unsigned int win) {
@@ -71,10 +72,10 @@ __global__ void encounter_trace_kernel(TPArrays tp, const int *__restrict__ tp_s
   double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
   float f[NM][SC_IN];
   const double T = sc.h * (double)C;
+  const double *vsys = vs0 + (size_t)s * 3 * NM;
   for (int q = 0; q < NM; ++q)
-    scorer_features(x, y, z, vx, vy, vz, xsys[3 * q], xsys[3 * q + 1], xsys[3 * q + 2],
-                    rsys[R_W(NM) + 3 * q], rsys[R_W(NM) + 3 * q + 1], rsys[R_W(NM) + 3 * q + 2],
-                    sqrt(prm[P_RC2(NM) + q]), T, f[q]);
+    scorer_features(x, y, z, vx, vy, vz, xsys[3 * q], xsys[3 * q + 1], xsys[3 * q + 2], vsys[3 * q],
+                    vsys[3 * q + 1], vsys[3 * q + 2], scorer_rc(prm[P_RC2(NM) + q]), T, f[q]);
   unsigned int hit = 0;
   double ax, ay, az;
   accel_tp<NM>(x, y, z, xsys, prm, sc.eps2, ax, ay, az);
@@ -322,7 +323,68 @@ hipError_t launch_grad(int which, const float *w, const float *feat, const float
 std::vector<float> &host_weights(Ctx *c, int which) { return which == 0 ? c->w_kep : c->w_sc; }
 float *dev_weights(Ctx *c, int which) { return which == 0 ? c->d_nnw_kep : c->d_nnw_sc; }
 
+// Upper bound of the scorer's logit over a box of its features: interval
+// arithmetic through the hidden layer (tanh is increasing), in FP64.
+double scorer_box_bound(const std::vector<float> &w, const double *lo, const double *hi) {
+  double out = w[SC_B2];
+  for (int u = 0; u < SC_H; ++u) {
+    double a = w[SC_B1 + u], b = w[SC_B1 + u];
+    for (int k = 0; k < SC_IN; ++k) {
+      const double wk = w[SC_W1 + u * SC_IN + k];
+      a += std::min(wk * lo[k], wk * hi[k]);
+      b += std::max(wk * lo[k], wk * hi[k]);
+    }
+    const double w2 = w[SC_W2 + u];
+    out += std::max(w2 * std::tanh(a), w2 * std::tanh(b));
+  }
+  return out;
+}
+
+// The same, tightened by splitting the box along its widest side (at most
+// `depth` times on any path) until a part's bound is `target` or below.
+double scorer_bound_split(const std::vector<float> &w, double *lo, double *hi, double target, int depth) {
+  const double u = scorer_box_bound(w, lo, hi);
+  if (u <= target || depth == 0) return u;
+  int k = 0;
+  for (int q = 1; q < SC_IN; ++q)
+    if (hi[q] - lo[q] > hi[k] - lo[k]) k = q;
+  const double l = lo[k], h = hi[k], m = 0.5 * (l + h);
+  hi[k] = m;
+  const double u1 = scorer_bound_split(w, lo, hi, target, depth - 1);
+  hi[k] = h;
+  lo[k] = m;
+  const double u2 = u1 > target ? u1 : scorer_bound_split(w, lo, hi, target, depth - 1);
+  lo[k] = l;
+  return std::max(u1, u2);
+}
+
 }  // namespace
+
+// The scorer's far-pair cutoff for weights w and threshold t: the smallest c
+// of a ladder such that every pair with both distance features at or above c
+// (distance and straight-line closest approach at least 8 c r_c; the other
+// features anywhere in their clamped ranges) has a logit bound below t, with
+// a margin far above what the device's FP32 sums and its tanh (about 1e-7
+// absolute each) can add; such pairs are left out of the scoring.  *bound
+// receives the bound at that c.  No safe c: returns +inf (nothing is left out).
+float scorer_cutoff(const std::vector<float> &w, float t, float *bound) {
+  static const double ladder[] = {1.5, 2.0, 3.0, 4.0, 6.0, 8.0};
+  double wsum = 1.0;
+  for (int u = 0; u < SC_H; ++u) wsum += std::fabs(w[SC_W2 + u]);
+  const double margin = 1e-3 * wsum;
+  *bound = INFINITY;
+  for (double c : ladder) {
+    double lo[SC_IN] = {c, -4.0, 0.0, c, 0.0, 0.0};
+    double hi[SC_IN] = {8.0, 4.0, 4.0, 8.0, 4.0, 2.0};
+    const double b = scorer_bound_split(w, lo, hi, (double)t - margin, 12) + margin;
+    if (b < (double)t) {
+      *bound = (float)b;
+      return (float)c;
+    }
+  }
+  return INFINITY;
+}
+
 }  // namespace mint
 
 using namespace mint;
@@ -345,9 +407,8 @@ int mint_nn_set_weights(void *ctx, int32_t which, const float *w) {
   if (!c || !w || nparams(which) < 0) return MINT_EINVAL;
   auto &hw = host_weights(c, which);
   hw.assign(w, w + nparams(which));
-  for (auto &g : c->graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
+  if (which == 1) c->sc_cut = scorer_cutoff(hw, c->score_thresh, &c->sc_sat_bound);
+  c->drop_graphs();
   if (c->uploaded && dev_weights(c, which)) {  // (the many-body engine keeps no network buffers)
     hipSetDevice(c->cfg.device);
     return c->hip(hipMemcpy(dev_weights(c, which), hw.data(), hw.size() * sizeof(float), hipMemcpyHostToDevice),
@@ -360,9 +421,22 @@ int mint_set_score_threshold(void *ctx, float t) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
   c->score_thresh = t;
-  for (auto &g : c->graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
+  if (!c->w_sc.empty()) c->sc_cut = scorer_cutoff(c->w_sc, t, &c->sc_sat_bound);
+  c->drop_graphs();
+  return MINT_OK;
+}
+
+int mint_scorer_skip(void *ctx, int32_t mode, float *bound, float *cutoff, int32_t *active) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  if (mode != -1 && mode != 0 && mode != 1) return c->fail(MINT_EINVAL, "mint_scorer_skip: mode must be -1, 0 or 1");
+  if (mode >= 0 && mode != c->scorer_skip_mode) {
+    c->scorer_skip_mode = mode;
+    c->drop_graphs();
+  }
+  if (bound) *bound = c->sc_sat_bound;
+  if (cutoff) *cutoff = c->sc_cut;
+  if (active) *active = c->scorer_skip_on() ? 1 : 0;
   return MINT_OK;
 }
 
@@ -445,18 +519,18 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
     hipMemcpyAsync(cmx.p, c->d_mx, mb, hipMemcpyDeviceToDevice, c->stream);
     hipMemcpyAsync(cmv.p, c->d_mv, mb, hipMemcpyDeviceToDevice, c->stream);
     rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, C, c->d_prm, (double *)cmx.p, (double *)cmv.p, c->d_xs0,
-                                c->d_rec, c->d_status, nullptr, sc, c->cfg.central_mass),
+                                c->d_rec, nullptr, nullptr, c->d_status, nullptr, sc, c->cfg.central_mass),
                 "trace ephemeris");
     if (rc) return rc;
     const long long cnt = (c->Ntp + stride - 1) / stride;
     if (cnt > 0) {
       if (c->NM == 5)
         encounter_trace_kernel<5><<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(
-            c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_rec, C, sc,
+            c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_mv, c->d_rec, C, sc,
             (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap, (unsigned)w);
       else
         encounter_trace_kernel<2><<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(
-            c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_rec, C, sc,
+            c->tp_arrays(), c->d_tp_sys, c->Ntp, stride, c->d_prm, c->d_xs0, c->d_mv, c->d_rec, C, sc,
             (float *)df.p, (float *)dl.p, (unsigned int *)dc.p, (unsigned)cap, (unsigned)w);
       if ((rc = c->hip(hipGetLastError(), "encounter trace launch"))) return rc;
     }
@@ -577,9 +651,7 @@ int mint_train(void *ctx, int32_t which, int64_t n, const float *feat, const flo
   }
   hipStreamSynchronize(c->stream);
   hipMemcpy(hw.data(), dw.p, np * 4, hipMemcpyDeviceToHost);
-  for (auto &g : c->graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
+  c->drop_graphs();
   if (c->uploaded && dev_weights(c, which))
     hipMemcpy(dev_weights(c, which), hw.data(), np * 4, hipMemcpyHostToDevice);
   return c->hip(hipGetLastError(), "train");

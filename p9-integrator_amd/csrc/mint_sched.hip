@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -45,13 +46,16 @@ int enc_grid(const Ctx *c) {
 
 
 
-// Ephemeris of one launch window into buffer (xs0, rec) on stream st.
-int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, KTimer *kt) {
+// Ephemeris of one launch window into buffer (xs0, rec) on stream st; the
+// window's end state also goes to the snapshot `next` (positions, then
+// velocities).
+int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, double *next, KTimer *kt) {
   const StepConst sc = c->step_const();
   if (kt) kt->start();
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, c->d_status, stats,
-                                  sc, c->cfg.central_mass),
+  const size_t nb = (size_t)c->S * c->NM * 3;
+  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, next, next + nb,
+                                  c->d_status, stats, sc, c->cfg.central_mass),
                   "ephemeris launch");
   if (rc) return rc;
   if (kt) kt->stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
@@ -84,7 +88,7 @@ int window_event(Ctx *c, std::vector<hipEvent_t> &v, size_t i, hipEvent_t *out) 
 // w-2.  The last window's tails run on the context stream at the end.
 // Without pipelining (or when timing each kernel) the same launches run in
 // the same order on the context stream.
-int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTimer *kt) {
+int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTimer *kt, bool capturing) {
   std::vector<int> win;
   for (int64_t done = 0; done < nsteps;) {
     const int C = (int)std::min<int64_t>(sh.C, nsteps - done);
@@ -119,6 +123,9 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     int r = window_event(c, v, i, &e);
     return r ? r : c->hip(hipEventRecord(e, st), "event record");
   };
+  // collect_stats bit 2: the main kernels stamp their wavefronts' entry and
+  // exit (buffer sized by batched_setup), read by mint_step after the call
+  const bool ktime = (c->cfg.collect_stats & 4) && !kt && c->d_kstamp != nullptr;
   auto wait = [&](hipStream_t st, std::vector<hipEvent_t> &v, size_t i) -> int {
     if (!pipe) return MINT_OK;
     hipEvent_t e;
@@ -149,24 +156,33 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
     return MINT_OK;
   };
+  // the first window's start state for its queue head (later windows' come
+  // from the ephemeris of the window before)
+  const long long nb = (long long)S * c->NM * 3;
+  copy_f64_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, sm>>>(c->d_snap[0], c->d_snap[0] + nb, c->d_mx, c->d_mv,
+                                                                nb);
+  if ((rc = c->hip(hipGetLastError(), "snapshot launch"))) return rc;
   if ((rc = record(c->ev_side, 0, sm)) || (rc = wait(sp, c->ev_side, 0))) return rc;
   for (int w = 0; w < W; ++w) {
     const int p = w % 2, B = w % 3, C = win[w];
-    // ephemeris of window w
+    // ephemeris of window w (its end state is window w+1's snapshot, buffer
+    // (w+1) mod 3, last read by the queue head of window w-2)
     if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
-    if ((rc = window_eph(c, sp, C, xs[B], rec[B], kt))) return rc;
-    if ((rc = record(c->ev_eph, w, sp)) || (rc = wait(sm, c->ev_eph, w))) return rc;
-    // queue head of window w
+    if ((rc = window_eph(c, sp, C, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
+    if ((rc = record(c->ev_eph, w, sp))) return rc;
+    // queue head of window w, from the window-start snapshot: beside the
+    // window's ephemeris
     zero_u32_kernel<<<1, 64, 0, sm>>>(ctl[p], 4);
     zero_u32_kernel<<<(S + 255) / 256, 256, 0, sm>>>(need[p], S);
     if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
-    if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, xs[B], rec[B], C, sc, scw,
-                                        c->score_thresh, c->d_carry, c->d_pre, q[p], ctl[p], need[p],
+    if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, c->d_snap[B], c->d_snap[B] + nb, C, sc, scw,
+                                        c->score_thresh, c->scorer_skip_on() ? c->sc_cut : INFINITY, c->d_carry, c->d_pre, q[p], ctl[p], need[p],
                                         slist_head[p], stats),
                      "queue head launch")))
       return rc;
     copy_u32_kernel<<<1, 1, 0, sm>>>(ctl[p] + 2, ctl[p]);
     if ((rc = c->hip(hipGetLastError(), "queue head count launch"))) return rc;
+    if ((rc = wait(sm, c->ev_eph, w))) return rc;
     // queue stream: tails of window w-1, then the head of window w
     if ((rc = record(c->ev_fork, w, sm)) || (rc = wait(sq, c->ev_fork, w))) return rc;
     if (w >= 1 && (rc = serve(w - 1, true, sq))) return rc;
@@ -177,26 +193,35 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
       if (kt) kt->start();
       if ((rc = c->hip(c->L->tp_main(sm, guess, sh.P, sh.block, nblocks, c->d_tiles, c->ntiles, tpx, tp,
                                      c->d_prm, xs[B], rec[B], C, sc, nnw, c->d_pre, q[p], ctl[p], need[p],
-                                     slist_tail[p], w + 1 < W ? c->d_carry : nullptr, c->d_status, stats),
+                                     slist_tail[p], w + 1 < W ? c->d_carry : nullptr, c->d_status, stats,
+                                     ktime ? c->d_kstamp + (size_t)w * 2 * c->kstamp_wpl : nullptr),
                        "main launch")))
         return rc;
       if (kt) kt->stop(c->timing.ms_tp_main, c->timing.n_tp_main);
     }
     if ((rc = wait(sm, c->ev_join, w)) || (rc = record(c->ev_tp, w, sm))) return rc;
   }
+  if (ktime && c->ntiles > 0) c->kwin_pending = W;
   // the last window's tails
   return serve(W - 1, true, sm);
 }
 
-int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out) {
+// The cached graph of an nsteps call at shape sh, captured now if there is
+// none for the context's current constants (GraphSig).  *nwin receives the
+// number of launch windows in it.
+int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out, int *nwin) {
+  const GraphSig sig = c->graph_sig();
   for (auto &g : c->graphs)
-    if (g.nsteps == nsteps && g.block == sh.block && g.P == sh.P && g.C == sh.C) {
+    if (g.nsteps == nsteps && g.block == sh.block && g.P == sh.P && g.C == sh.C && g.sig.same(sig)) {
       *out = g.exec;
+      *nwin = g.nwin;
       return MINT_OK;
     }
   int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
-  rc = enqueue_batched(c, sh, nsteps, true, nullptr);
+  rc = enqueue_batched(c, sh, nsteps, true, nullptr, true);
+  const int W = c->kwin_pending;
+  c->kwin_pending = 0;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
@@ -208,6 +233,13 @@ int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out) {
   e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if ((rc = c->hip(e, "graph instantiate"))) return rc;
+  // make the graph resident on the device now, so its first replay costs no
+  // more than any other
+  if ((rc = c->hip(hipGraphUpload(ex, c->stream), "graph upload")) ||
+      (rc = c->hip(hipStreamSynchronize(c->stream), "graph upload"))) {
+    hipGraphExecDestroy(ex);
+    return rc;
+  }
   if (c->graphs.size() >= 8) {
     hipGraphExecDestroy(c->graphs.front().exec);
     c->graphs.erase(c->graphs.begin());
@@ -217,13 +249,18 @@ int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out) {
   ge.block = sh.block;
   ge.P = sh.P;
   ge.C = sh.C;
+  ge.nwin = W;
+  ge.sig = sig;
   ge.exec = ex;
   c->graphs.push_back(ge);
   *out = ex;
+  *nwin = W;
   return MINT_OK;
 }
 
-int run_batched(Ctx *c, int64_t nsteps) {
+// Shape checks and buffers of an nsteps batched call at the context's launch
+// shape (all allocation happens here, before any capture).
+int batched_setup(Ctx *c, int64_t nsteps, Shape *out) {
   const Shape sh = batched_shape(c);
   if (!(sh.P == 1 || sh.P == 2 || sh.P == 4)) return c->fail(MINT_EINVAL, "tps_per_lane must be 1, 2 or 4");
   if (sh.block < 64 || sh.block > 512 || (sh.block & 63)) return c->fail(MINT_EINVAL, "block_size must be a multiple of 64 in [64,512]");
@@ -232,14 +269,50 @@ int run_batched(Ctx *c, int64_t nsteps) {
   if (rc) return rc;
   if ((rc = c->ensure_rec2(sh.C))) return rc;
   if ((rc = c->ensure_tiles(sh.block, sh.P))) return rc;
+  if (c->cfg.collect_stats & 4) {
+    // main-kernel launch stamps: two per wavefront per launch window
+    const int wpl = 8 * ((c->ntiles + 7) / 8) * (sh.block / 64);
+    const size_t need = (size_t)((nsteps + sh.C - 1) / sh.C) * 2 * (size_t)std::max(wpl, 1);
+    if (need > c->kstamp_cap || wpl != c->kstamp_wpl) {
+      if (c->d_kstamp) {  // a captured graph holds the old pointer
+        c->drop_graphs();
+        hipFree(c->d_kstamp);
+        c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void *)c->d_kstamp));
+        c->d_kstamp = nullptr;
+        c->kstamp_cap = 0;
+      }
+      const size_t cap = std::max(need, c->kstamp_cap);
+      if ((rc = c->dalloc((void **)&c->d_kstamp, cap * sizeof(long long), "launch stamps"))) return rc;
+      c->kstamp_cap = cap;
+      c->kstamp_wpl = wpl;
+    }
+    if (c->wall_khz <= 0.0) {
+      int khz = 0;
+      if ((rc = c->hip(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->cfg.device), "wall clock rate")))
+        return rc;
+      c->wall_khz = (double)khz;
+    }
+  }
+  *out = sh;
+  return MINT_OK;
+}
+
+int run_batched(Ctx *c, int64_t nsteps) {
+  Shape sh;
+  int rc = batched_setup(c, nsteps, &sh);
+  if (rc) return rc;
+  c->kwin_pending = 0;
   if (timing_on(c)) {
     KTimer kt(c);
-    return enqueue_batched(c, sh, nsteps, false, &kt);
+    return enqueue_batched(c, sh, nsteps, false, &kt, false);
   }
-  if (!c->cfg.use_graph) return enqueue_batched(c, sh, nsteps, true, nullptr);
+  if (!c->cfg.use_graph) return enqueue_batched(c, sh, nsteps, true, nullptr, false);
   hipGraphExec_t ex;
-  if ((rc = graph_for(c, sh, nsteps, &ex))) return rc;
-  return c->hip(hipGraphLaunch(ex, c->stream), "graph launch");
+  int W = 0;
+  if ((rc = graph_for(c, sh, nsteps, &ex, &W))) return rc;
+  if ((rc = c->hip(hipGraphLaunch(ex, c->stream), "graph launch"))) return rc;
+  if (c->cfg.collect_stats & 4) c->kwin_pending = W;
+  return MINT_OK;
 }
 
 // Plain (groups of systems_per_launch systems) and flat (all systems): one
@@ -259,7 +332,7 @@ int run_per_step(Ctx *c, int64_t nsteps, bool flat) {
       const int s1 = std::min(c->S, s0 + spl);
       kt.start();
       rc = c->hip(c->L->ephemeris(c->stream, s0, s1, 1, c->d_prm, c->d_mx, c->d_mv, c->d_xs0, c->d_rec,
-                                  c->d_status, stats, sc, c->cfg.central_mass),
+                                  nullptr, nullptr, c->d_status, stats, sc, c->cfg.central_mass),
                   "ephemeris launch");
       if (rc) return rc;
       kt.stop(c->timing.ms_ephemeris, c->timing.n_ephemeris);
@@ -308,6 +381,7 @@ int mint_step(void *ctx, int64_t nsteps) {
   if (rc) return rc;
   rc = step_impl(c, nsteps);
   if (rc) {
+    c->kwin_pending = 0;
     hipStreamSynchronize(c->stream);
     return rc;
   }
@@ -316,7 +390,41 @@ int mint_step(void *ctx, int64_t nsteps) {
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->timing.ms_total += ms;
+  if (c->kwin_pending > 0) {
+    // each launch: its last wavefront's exit minus its first wavefront's entry
+    const size_t per = 2 * (size_t)c->kstamp_wpl;
+    std::vector<long long> h((size_t)c->kwin_pending * per);
+    rc = c->hip(hipMemcpy(h.data(), c->d_kstamp, h.size() * sizeof(long long), hipMemcpyDeviceToHost),
+                "D2H launch stamps");
+    for (int w = 0; w < c->kwin_pending && !rc; ++w) {
+      long long t0 = LLONG_MAX, t1 = LLONG_MIN;
+      for (size_t i = 0; i < per; i += 2) {
+        t0 = std::min(t0, h[w * per + i]);
+        t1 = std::max(t1, h[w * per + i + 1]);
+      }
+      c->timing.ms_tp_main += (double)(t1 - t0) / c->wall_khz;
+      c->timing.n_tp_main += 1;
+    }
+    c->kwin_pending = 0;
+    if (rc) return rc;
+  }
   return c->hip(hipGetLastError(), "step");
+}
+
+int mint_prepare(void *ctx, int64_t nsteps) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c) return MINT_EINVAL;
+  if (!c->uploaded) return c->fail(MINT_ESTATE, "mint_prepare: nothing uploaded");
+  if (nsteps < 0) return c->fail(MINT_EINVAL, "mint_prepare: negative step count");
+  if (nsteps == 0 || c->many || c->cfg.form != MINT_FORM_BATCHED || !c->cfg.use_graph || timing_on(c))
+    return MINT_OK;
+  hipSetDevice(c->cfg.device);
+  Shape sh;
+  int rc = batched_setup(c, nsteps, &sh);
+  if (rc) return rc;
+  hipGraphExec_t ex;
+  int W = 0;
+  return graph_for(c, sh, nsteps, &ex, &W);
 }
 
 // ---------------------------------------------------------------------------
@@ -428,9 +536,7 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   restore();
   hipDeviceSynchronize();
   for (void *p : save) hipFree(p);
-  for (auto &g : c->graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
+  c->drop_graphs();
   if (rc) return rc;
   c->tuned_block = bs.block;
   c->tuned_P = bs.P;

@@ -103,6 +103,7 @@ _SIGS = {
     "mint_last_error": (C.c_char_p, [C.c_void_p]),
     "mint_upload": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_step": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mint_prepare": (C.c_int, [C.c_void_p, C.c_int64]),
     "mint_download": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_energy": (C.c_int, [C.c_void_p, _DP]),
     "mint_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
@@ -129,6 +130,8 @@ _SIGS = {
     "mint_nn_get_weights": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "mint_nn_set_weights": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "mint_set_score_threshold": (C.c_int, [C.c_void_p, C.c_float]),
+    "mint_scorer_skip": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_int32)]),
     "mint_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "mint_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
     "mint_set_radii": (C.c_int, [C.c_void_p, _DP]),
@@ -280,6 +283,10 @@ class Mint:
     def step(self, nsteps: int):
         self._chk(self.L.mint_step(self.h, int(nsteps)), "mint_step")
 
+    def prepare(self, nsteps: int):
+        """Build the launch graph of an nsteps call ahead of it (host work only)."""
+        self._chk(self.L.mint_prepare(self.h, int(nsteps)), "mint_prepare")
+
     def download(self) -> State:
         S, NM, off = self._shape
         n = int(off[-1])
@@ -397,6 +404,17 @@ class Mint:
         w = np.ascontiguousarray(w, np.float32)
         assert len(w) == self.L.mint_nn_nparams(which)
         self._chk(self.L.mint_nn_set_weights(self.h, which, _f(w)), "mint_nn_set_weights")
+
+    def scorer_skip(self, mode=-1):
+        """Set the far-pair skip of the scorer (1 where provably neutral, 0
+        never, -1 unchanged); returns (interval bound of the logit over the
+        far pairs, the cutoff c in units of 8 r_c, whether pairs are skipped)."""
+        b = C.c_float()
+        cut = C.c_float()
+        a = C.c_int32()
+        self._chk(self.L.mint_scorer_skip(self.h, int(mode), C.byref(b), C.byref(cut), C.byref(a)),
+                  "mint_scorer_skip")
+        return b.value, cut.value, bool(a.value)
 
     def set_score_threshold(self, t):
         self._chk(self.L.mint_set_score_threshold(self.h, float(t)), "mint_set_score_threshold")

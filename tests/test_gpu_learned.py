@@ -144,3 +144,37 @@ def test_scorer_moves_encounters_out_of_the_main_kernel():
     assert s0["deferred"] > 30
     assert s1["deferred"] <= 0.2 * s0["deferred"]
     assert np.array_equal(o0.tx, o1.tx) and np.array_equal(o0.tv, o1.tv)
+
+
+def test_scorer_far_pair_bound_covers_the_network_and_skipping_changes_no_pick():
+    """Far pairs are left out of the scorer's work only under a proof: the
+    library's interval bound of the logit over every pair with both distance
+    features at or above its cutoff (checked here against 2e5 sampled pairs
+    of that box, corners included, on the FP64 reference network) lies below
+    the threshold.  With the trained weights the skip is on, and on a sweep
+    slice it picks exactly the TPs the full scoring picks, with bitwise equal
+    results."""
+    st = W.p9_sweep(systems=np.arange(0, 4096, 32), tp_scale=1 / 2)
+    res = []
+    for mode in (0, 1):
+        with mint.Mint(dt=0.5, use_scorer=1, collect_stats=1, chunk_steps=T.window(), block_size=256,
+                       tps_per_lane=1) as g:
+            thr = _weights(g)
+            bound, cut, active = g.scorer_skip(mode)
+            w = g.nn_weights(mint.NET_SCORER).astype(np.float64)
+            g.upload(st)
+            g.step(2 * T.window())
+            res.append((g.download(), g.stats(), bound, cut, active))
+    (o0, s0, b0, c0, a0), (o1, s1, b1, c1, a1) = res
+    assert not a0 and a1 and b0 == b1 and c0 == c1 <= 8.0 and b1 < thr
+    rng = np.random.default_rng(4)
+    lo = np.array([c1, -4.0, 0.0, c1, 0.0, 0.0])
+    hi = np.array([8.0, 4.0, 4.0, 8.0, 4.0, 2.0])
+    X = rng.uniform(lo, hi, size=(200000, 6))
+    for k in range(6):  # faces of the box too
+        X[k::13, k] = rng.choice([lo[k], hi[k]], size=X[k::13].shape[0])
+    assert np.max(N.forward(1, w, X)) < b1
+    print(f"far-pair cutoff {c1} (pairs beyond {8 * c1} r_c), bound {b1:.3f} < threshold {thr:.3f}; prebucketed "
+          f"{s0['prebucketed']} (all pairs) vs {s1['prebucketed']} (far pairs left out)", flush=True)
+    assert s0["prebucketed"] == s1["prebucketed"] > 0
+    assert np.array_equal(o0.tx, o1.tx) and np.array_equal(o0.tv, o1.tv)
