@@ -1,11 +1,18 @@
-"""Summarise rocprofv3 result databases (rocpd SQLite) as text.
+"""Summarise rocprofv3 results as text.
 
-    python prof_summary.py trace  <trace_results.db>            per-kernel stats
+    python prof_summary.py trace  <trace_results.db>            per-kernel stats (rocpd SQLite)
     python prof_summary.py counter <results.db> [kernel-substring]   per-kernel
-        mean of each collected counter per dispatch
+        mean of each collected counter per dispatch (rocpd SQLite)
+    python prof_summary.py csvtrace <kernel_trace.csv> [kernel-substring]
+        per-kernel stats from a CSV kernel trace, and the duration of every
+        dispatch of the kernels matching the substring
+    python prof_summary.py csvcounter <counter_collection.csv> [kernel-substring]
+        every dispatch's counter values (CSV), for the kernels matching
 """
+import csv
 import sqlite3
 import sys
+from collections import defaultdict
 
 
 def short(name):
@@ -36,8 +43,41 @@ def counter(db, filt=""):
             print(f"| `{short(k)}` | {c} | {v:.6g} | {n} |")
 
 
+def csvtrace(path, filt=""):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    agg = defaultdict(list)
+    res = {}
+    for r in rows:
+        k = short(r["Kernel_Name"])
+        agg[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+        res[k] = (r["VGPR_Count"], r["SGPR_Count"], r["Scratch_Size"], r["LDS_Block_Size"])
+    tot = sum(sum(v) for v in agg.values())
+    print("| kernel | calls | total ms | avg ms | % | vgpr / sgpr / scratch B / LDS B |")
+    print("|---|---|---|---|---|---|")
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| `{k}` | {len(v)} | {sum(v):.3f} | {sum(v) / len(v):.4f} | {100 * sum(v) / tot:.2f} | "
+              f"{' / '.join(res[k])} |")
+    if filt:
+        print()
+        print(f"dispatches of kernels matching `{filt}` (ms, in order):")
+        for k, v in agg.items():
+            if filt in k:
+                print(f"  `{k}`: " + ", ".join(f"{x:.4f}" for x in v))
+
+
+def csvcounter(path, filt=""):
+    rows = list(csv.DictReader(open(path)))
+    per = defaultdict(dict)
+    for r in rows:
+        if filt in r["Kernel_Name"]:
+            per[(int(r["Dispatch_Id"]), short(r["Kernel_Name"]))][r["Counter_Name"]] = float(r["Counter_Value"])
+    for (d, k), cs in sorted(per.items()):
+        print(f"dispatch {d} `{k}`: " + ", ".join(f"{c} {v:.6g}" for c, v in sorted(cs.items())))
+
+
 if __name__ == "__main__":
-    if sys.argv[1] == "trace":
-        trace(sys.argv[2])
-    else:
-        counter(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
+    cmd, path = sys.argv[1], sys.argv[2]
+    filt = sys.argv[3] if len(sys.argv) > 3 else ""
+    {"trace": lambda: trace(path), "counter": lambda: counter(path, filt), "csvtrace": lambda: csvtrace(path, filt),
+     "csvcounter": lambda: csvcounter(path, filt)}[cmd]()
