@@ -173,6 +173,23 @@ int mint_step(void *ctx, int64_t nsteps);
  * directly, src/mercurius.cu:184-196). */
 int mint_prepare(void *ctx, int64_t nsteps);
 
+/* Allocation-free variant of mint_step (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Advances the
+ * resident batch by nsteps with the flat per-step schedule (per step one
+ * ephemeris launch and one TP step launch over all systems: the kernels and
+ * results of mint_step with form MINT_FORM_FLAT) and keeps the per-step
+ * scratch (step-start massive positions and one ephemeris record per system)
+ * in a fixed 8 MiB device array of the library, held on every GPU the
+ * library runs on until the process ends, instead of context buffers.  The
+ * batch must fit it: S * (3 n_massive + 2 ceil((6 + 9 n_massive) / 2))
+ * doubles at most 2^20 (4096 systems of 5 massive bodies need 274,432),
+ * else MINT_ELIMIT; systems of more than 8 massive bodies are refused
+ * (MINT_EINVAL).  Calls no hipMalloc or hipFree.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (from several contexts or
+ * threads) race. */
+int mint_step_noalloc(void *ctx, int64_t nsteps);
+
 /* Copy the state back, converting velocities to heliocentric form (replaces
  * the teardown src/sim.cuh:744-765 and the D2H copies src/mercurius.cu:203-243).
  * Only the non-NULL arrays of st are written; shapes must match the upload. */

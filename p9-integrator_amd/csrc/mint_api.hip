@@ -331,6 +331,7 @@ int mint_create(const mint_config *cfg, void **out) {
   }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
+  noalloc_init(k.device);  // mint_step_noalloc's scratch address on this device, once per device
   *out = c;
   return MINT_OK;
 }

@@ -191,6 +191,7 @@ int mb_set_radii(Ctx *c);   // body parameters again after a change of rad
 int run_many(Ctx *c, int64_t nsteps);
 
 extern std::string g_create_error;
+void noalloc_init(int device);  // mint_noalloc.hip
 float scorer_cutoff(const std::vector<float> &w, float t, float *bound);
 const char *form_name(int f);
 

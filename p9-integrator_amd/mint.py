@@ -104,6 +104,7 @@ _SIGS = {
     "mint_upload": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "mint_prepare": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mint_step_noalloc": (C.c_int, [C.c_void_p, C.c_int64]),
     "mint_download": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_energy": (C.c_int, [C.c_void_p, _DP]),
     "mint_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
@@ -282,6 +283,11 @@ class Mint:
 
     def step(self, nsteps: int):
         self._chk(self.L.mint_step(self.h, int(nsteps)), "mint_step")
+
+    def step_noalloc(self, nsteps: int):
+        """mint_step_noalloc: the flat per-step schedule with the library's
+        fixed device scratch (no allocation)."""
+        self._chk(self.L.mint_step_noalloc(self.h, int(nsteps)), "mint_step_noalloc")
 
     def prepare(self, nsteps: int):
         """Build the launch graph of an nsteps call ahead of it (host work only)."""
