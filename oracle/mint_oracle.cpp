@@ -545,7 +545,10 @@ std::vector<std::vector<int>> encounter_groups(const Sys &s, int64_t &overlaps) 
 }
 
 // --- one Mercurius step of one system (src/sim.cuh:679-742) -------------------
-uint32_t step(Sys &s, const Params &p, double M, Stats &stt) {
+// hits (trace mode): per TP of the system, OR of the bodies whose exact check
+// fired; the TP then takes its Kepler drift whatever the check says (the
+// library's encounter-trace definition).
+uint32_t step(Sys &s, const Params &p, double M, Stats &stt, uint32_t *hits = nullptr) {
   const int nm = s.nm;
   const double hh = 0.5 * p.h;
   uint32_t st = 0;
@@ -578,12 +581,21 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt) {
   }
   // 3. encounter flags on post-jump positions; 4. drift
   const std::vector<Body> pert0 = s.mb;
+  size_t ti = 0;
   for (Body &t : s.tp) {
     bool flag = false;
+    uint32_t hb = 0;
     for (int j = 0; j < nm; ++j) {
       const V3 d = sub(pert0[j].x, t.x);
-      flag = flag || (dot(d, d) < s.rc_tp[j].r2);
+      const bool in = dot(d, d) < s.rc_tp[j].r2;
+      flag = flag || in;
+      if (in) hb |= 1u << j;
     }
+    if (hits) {
+      hits[ti] |= hb;
+      flag = false;
+    }
+    ++ti;
     if (flag) {
       st |= bs_tp(t, pert0, s, p, stt.bs_rhs);
       stt.bs_steps++;
@@ -710,9 +722,66 @@ int orc_bs_tp(double *y6, int nm, const double *m, const double *rc, const doubl
 // velocities are moved to democratic-heliocentric form before the first step
 // and back after the last (src/sim.cuh:555-587,677,748).  rc[S*NM] are the
 // changeover radii.  threads > 1 spreads systems over std::thread.
+static int run_impl(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, const double *m,
+                    const double *rc, const double *rad, double *mx, double *mv, double *tx, double *ty, double *tz,
+                    double *tvx, double *tvy, double *tvz, int64_t nsteps, uint32_t *status, OrcStats *stats,
+                    uint32_t *hits);
+
 int orc_run(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, const double *m, const double *rc,
             const double *rad, double *mx, double *mv, double *tx, double *ty, double *tz, double *tvx,
             double *tvy, double *tvz, int64_t nsteps, uint32_t *status, OrcStats *stats) {
+  return run_impl(op, S, NM, tp_off, m, rc, rad, mx, mv, tx, ty, tz, tvx, tvy, tvz, nsteps, status, stats, nullptr);
+}
+
+// Encounter-trace labels: nsteps steps in which every TP follows its Kepler
+// drift; hits[i] receives the OR over the steps of the bodies whose exact
+// check fired for TP i (zeroed here).  The state is advanced like orc_run's.
+int orc_trace_hits(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, const double *m,
+                   const double *rc, double *mx, double *mv, double *tx, double *ty, double *tz, double *tvx,
+                   double *tvy, double *tvz, int64_t nsteps, uint32_t *hits) {
+  std::memset(hits, 0, sizeof(uint32_t) * (size_t)tp_off[S]);
+  return run_impl(op, S, NM, tp_off, m, rc, nullptr, mx, mv, tx, ty, tz, tvx, tvy, tvz, nsteps, nullptr, nullptr,
+                  hits);
+}
+
+// One Kepler-trace sample (the library's trace definition) of a drift by dt
+// from the DH state (x, v): features f[4] (log2 of the dimensionless step
+// tau = dt sqrt(mu / r0^3), zeta0 / mu, eta0 / sqrt(mu r0), r0 beta / mu
+// clamped to +-2) and target c = (s* / s_series - 1) / tau^3, s* the root of
+// this oracle's own converged solve from the series guess.  Returns 0 where
+// the trace takes no sample (unbound orbit, no convergence, non-finite c).
+int orc_kepler_sample(const double *x, const double *v, double mu, double dt, float *f, float *c) {
+  Body b{{x[0], x[1], x[2]}, {v[0], v[1], v[2]}};
+  const double r0 = std::sqrt(dot(b.x, b.x));
+  const double inv_r0 = 1.0 / r0;
+  const double v2 = dot(b.v, b.v);
+  const double eta0 = dot(b.x, b.v);
+  const double beta = std::fma(2.0 * mu, inv_r0, -v2);
+  const double zeta0 = std::fma(-beta, r0, mu);
+  if (!(beta > 0.0)) return 0;
+  const double u = dt * inv_r0;
+  const double e = eta0 * inv_r0;
+  const double s0 = u * std::fma(u, std::fma(u, std::fma(0.5 * e, e, -(zeta0 * inv_r0) * (1.0 / 6.0)), -0.5 * e), 1.0);
+  bool ok = false;
+  double sroot = 0.0;
+  kepler_drift(b, mu, dt, 1, ok, &sroot);
+  if (!ok) return 0;
+  const double tau = dt * std::sqrt(mu / (r0 * (r0 * r0)));
+  const double cc = (sroot / s0 - 1.0) / (tau * (tau * tau));
+  if (!std::isfinite(cc)) return 0;
+  f[0] = (std::log2((float)tau) + 6.0f) * 0.25f;
+  f[1] = (float)(zeta0 / mu);
+  f[2] = (float)(eta0 / std::sqrt(mu * r0));
+  const float ra = (float)(r0 * beta / mu);
+  f[3] = std::fmin(std::fmax(ra, -2.0f), 2.0f);
+  *c = (float)cc;
+  return 1;
+}
+
+static int run_impl(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, const double *m,
+                    const double *rc, const double *rad, double *mx, double *mv, double *tx, double *ty, double *tz,
+                    double *tvx, double *tvy, double *tvz, int64_t nsteps, uint32_t *status, OrcStats *stats,
+                    uint32_t *hits) {
   if (NM > MAXB && tp_off[S] > 0) return -1;  // test particles need at most MAXB massive bodies
   const double M = op->central_mass;
   const Params p{op->dt, M, op->softening2, op->bs_tol, op->guess};
@@ -744,7 +813,7 @@ int orc_run(const OrcParams *op, int32_t S, int32_t NM, const int64_t *tp_off, c
       for (int64_t i = tp_off[si]; i < tp_off[si + 1]; ++i)
         s.tp.push_back({{tx[i], ty[i], tz[i]}, {tvx[i] - sh.x, tvy[i] - sh.y, tvz[i] - sh.z}});
       uint32_t st = 0;
-      for (int64_t n = 0; n < nsteps; ++n) st |= step(s, p, M, tstats[tid]);
+      for (int64_t n = 0; n < nsteps; ++n) st |= step(s, p, M, tstats[tid], hits ? hits + tp_off[si] : nullptr);
       // back to heliocentric: v + (sum m v_dh)/M
       double qx = 0, qy = 0, qz = 0;
       for (int j = 0; j < NM; ++j) {

@@ -55,6 +55,12 @@ def lib():
         L.orc_encounter_groups.restype = C.c_int
         L.orc_accel_tp.argtypes = [C.c_int32, _DP, _DP, _DP, _DP, C.c_double, _DP]
         L.orc_accel_tp.restype = C.c_int
+        L.orc_trace_hits.argtypes = [C.POINTER(OrcParams), C.c_int32, C.c_int32, C.POINTER(C.c_int64), _DP, _DP,
+                                     _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int64, C.POINTER(C.c_uint32)]
+        L.orc_trace_hits.restype = C.c_int
+        L.orc_kepler_sample.argtypes = [_DP, _DP, C.c_double, C.c_double, C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float)]
+        L.orc_kepler_sample.restype = C.c_int
         _L = L
     return _L
 
@@ -130,6 +136,37 @@ def run(state, nsteps, dt, central_mass=1.0, softening2=1e-8, bs_tol=1e-12, gues
         raise ValueError("oracle: test particles need at most 8 massive bodies")
     out = dict(mx=mx, mv=mv, tx=np.stack(t[:3], 1), tv=np.stack(t[3:], 1), rc=rc)
     return out, status, {n: getattr(stats, n) for n, _ in OrcStats._fields_}
+
+
+def trace_hits(state, nsteps, dt, central_mass=1.0, softening2=1e-8, bs_tol=1e-12, n_hill=3.0):
+    """Per TP, the bitmask of bodies whose exact check fires in `nsteps`
+    steps along the TP's Kepler path (the encounter-trace labels)."""
+    S, NM = state.m.shape
+    off = np.ascontiguousarray(state.tp_offset, np.int64)
+    m = np.ascontiguousarray(state.m, np.float64)
+    rc = state.rc if state.rc is not None else hill_radii(central_mass, n_hill, state.m, state.mx, state.mv).reshape(S, NM)
+    rc = np.ascontiguousarray(rc, np.float64)
+    mx = np.array(state.mx, dtype=np.float64, order="C")
+    mv = np.array(state.mv, dtype=np.float64, order="C")
+    t = [np.ascontiguousarray(state.tx[:, d], np.float64).copy() for d in range(3)] + \
+        [np.ascontiguousarray(state.tv[:, d], np.float64).copy() for d in range(3)]
+    hits = np.zeros(int(off[-1]), np.uint32)
+    p = OrcParams(dt, central_mass, softening2, bs_tol, 1, 1)
+    lib().orc_trace_hits(C.byref(p), S, NM, off.ctypes.data_as(C.POINTER(C.c_int64)), _d(m), _d(rc), _d(mx), _d(mv),
+                         *[_d(a) for a in t], int(nsteps), hits.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return hits
+
+
+def kepler_sample(x, v, mu, dt):
+    """(features[4], target) of the Kepler trace for a drift from the DH
+    state (x, v), or None where the trace takes no sample."""
+    x = np.ascontiguousarray(x, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    f = np.zeros(4, np.float32)
+    c = C.c_float()
+    if not lib().orc_kepler_sample(_d(x), _d(v), mu, dt, f.ctypes.data_as(C.POINTER(C.c_float)), C.byref(c)):
+        return None
+    return f, np.float32(c.value)
 
 
 def energy(m, mx, mv, M=1.0, eps2=1e-8):
