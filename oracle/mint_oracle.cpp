@@ -51,6 +51,7 @@ constexpr int NEWTON_MAXIT = 40;
 struct Stats {
   int64_t tp_steps = 0, kepler = 0, newton = 0, bs_steps = 0, bs_rhs = 0;
   int64_t mgroups = 0, mgroup_bodies = 0, overlaps = 0;
+  int64_t mkepler = 0, mnewton = 0;  // massive bodies' Kepler drifts and their Newton iterations
 };
 
 // --- Stumpff functions (Danby 1988 §6.9), replacing the Kepler-equation
@@ -623,7 +624,8 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt, uint32_t *hits = nu
   for (int j = 0; j < nm; ++j) {
     if (ingroup[j]) continue;
     bool ok;
-    kepler_drift(s.mb[j], p.mu, p.h, 1, ok);
+    stt.mnewton += kepler_drift(s.mb[j], p.mu, p.h, 1, ok);
+    stt.mkepler++;
     if (!ok) st |= 2u;
   }
   // 5. jump 1/2 with the momentum after the drift
@@ -668,6 +670,7 @@ struct OrcParams {
 struct OrcStats {
   int64_t tp_steps, kepler_solves, newton_iters, bs_steps, bs_rhs_evals;
   int64_t massive_groups, massive_group_bodies, overlaps;
+  int64_t massive_kepler_solves, massive_newton_iters;
 };
 
 int orc_stumpff(double z, double *c) {
@@ -861,6 +864,8 @@ static int run_impl(const OrcParams *op, int32_t S, int32_t NM, const int64_t *t
       stats->massive_groups += s.mgroups;
       stats->massive_group_bodies += s.mgroup_bodies;
       stats->overlaps += s.overlaps;
+      stats->massive_kepler_solves += s.mkepler;
+      stats->massive_newton_iters += s.mnewton;
     }
   }
   return 0;

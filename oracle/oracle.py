@@ -22,7 +22,8 @@ class OrcParams(C.Structure):
 
 class OrcStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("tp_steps", "kepler_solves", "newton_iters", "bs_steps", "bs_rhs_evals",
-                                         "massive_groups", "massive_group_bodies", "overlaps")]
+                                         "massive_groups", "massive_group_bodies", "overlaps",
+                                         "massive_kepler_solves", "massive_newton_iters")]
 
 
 _L = None

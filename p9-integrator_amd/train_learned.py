@@ -5,13 +5,13 @@ the weights.
     python train_learned.py [--out weights/learned.json] [--kepler-samples N]
 
 Kepler warm-start MLP: samples (features, c) of drifts, c = (s*/s_series - 1)
-/ tau^3 with s* the converged universal variable.  Traced from the Planet Nine
-sweep and the single-system P9 workload (their test particles and massive
-bodies) and from an inner-belt workload; only samples with tau above
-TRAIN_TAU_MIN are kept, because the library consults the network only where
-tau exceeds 0.02 (below that the series guess already converges in one Newton
-iteration).  The target is standardised for training and the scale folded back
-into the output layer.
+/ tau^3 with s* the converged universal variable.  At least 1e7 are traced
+from the runs of the 4096-system Planet Nine sweep and of the single P9
+system (their giants' drifts: every test particle there has tau < 0.004,
+where c is below the rounding of s*), plus an inner-belt workload, the
+regime where the library's gate (tau > 0.02) opens; only samples with tau
+above TRAIN_TAU_MIN are kept.  The target is standardised for training and
+the scale folded back into the output layer.
 
 Encounter scorer: (pair features at a launch-window start, label = the exact
 check fired for that pair during the window), traced from the sweep (the trace
@@ -42,27 +42,46 @@ def log(*a):
     print(*a, flush=True)
 
 
-def kepler_traces(n_target, seed):
+def _traced(st, calls, cap, steps=16):
+    """Kepler samples with tau > TRAIN_TAU_MIN from `calls` traces of `steps`
+    steps each (every massive body and a stride of the TPs at 16 points)."""
     feats, tgts = [], []
-    jobs = [("p9_sweep[0:512]", W.p9_sweep(systems=np.arange(512)), 400),
-            ("p9_single", W.p9_single(100_000), 400),
-            ("inner_belt", W.inner_belt(200_000, (1.2, 4.0)), 200)]
-    for name, st, steps in jobs:
-        with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES) as g:
-            g.upload(st)
-            f, t = g.trace_kepler(steps, cap=min(n_target, 8_000_000))
-        tau = 2.0 ** (4 * f[:, 0] - 6)
-        keep = (tau > TRAIN_TAU_MIN) & np.isfinite(t)
-        log(f"kepler trace {name}: {len(t)} samples, {keep.sum()} with tau > {TRAIN_TAU_MIN}")
-        feats.append(f[keep])
-        tgts.append(t[keep])
-    f = np.concatenate(feats)
-    t = np.concatenate(tgts)
+    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, kepler_guess=mint.GUESS_SERIES) as g:
+        g.upload(st)
+        for _ in range(calls):
+            f, t = g.trace_kepler(steps, cap=cap)
+            tau = 2.0 ** (4 * f[:, 0] - 6)
+            keep = (tau > TRAIN_TAU_MIN) & np.isfinite(t)
+            feats.append(f[keep])
+            tgts.append(t[keep])
+    return np.concatenate(feats), np.concatenate(tgts)
+
+
+def kepler_traces(n_cfg, n_belt, seed):
+    """(features, c) samples.  From the runs of the single P9 system with 1e5
+    TPs and of the 4096-system sweep (the C1 and C3 workloads): at least
+    n_cfg with tau above TRAIN_TAU_MIN (their drifts of smaller tau, every TP
+    of those runs among them, carry a target below the rounding of s*: c =
+    (s*/s_series - 1)/tau^3 with tau^3 < 6e-8).  Those are the giants'
+    drifts, sampled every step of 16-step traces.  Plus n_belt from an
+    inner-belt workload (tau 0.08-0.3), where the library's gate opens."""
     rng = np.random.default_rng(seed)
-    if len(t) > n_target:
-        sel = rng.choice(len(t), n_target, replace=False)
-        f, t = f[sel], t[sel]
-    return f, t
+    sweep = W.p9_sweep(systems=np.arange(4096))
+    giants = mint.State(np.zeros(sweep.n_systems + 1, np.int64), sweep.m, sweep.mx, sweep.mv, np.zeros((0, 3)),
+                        np.zeros((0, 3)))
+    per_call = 16 * 3 * sweep.n_systems  # Jupiter, Saturn and Uranus pass the tau cut
+    f1, t1 = _traced(W.p9_single(100_000), 4, 400_000)
+    f3, t3 = _traced(giants, int(np.ceil(max(n_cfg - len(t1), 0) / per_call)) + 1, 2 * 16 * 5 * sweep.n_systems)
+    fb, tb = _traced(W.inner_belt(200_000, (1.2, 4.0)), 1, 8_000_000, steps=200)
+    if len(tb) > n_belt:
+        sel = rng.choice(len(tb), n_belt, replace=False)
+        fb, tb = fb[sel], tb[sel]
+    counts = {"p9_single (C1)": int(len(t1)), "p9_sweep_4096 (C3)": int(len(t3)), "inner_belt": int(len(tb))}
+    log(f"kepler traces with tau > {TRAIN_TAU_MIN}: {counts}")
+    f = np.concatenate([f1, f3, fb])
+    t = np.concatenate([t1, t3, tb])
+    perm = rng.permutation(len(t))
+    return f[perm], t[perm], counts
 
 
 WINDOW = 128  # launch window (steps) the scorer is trained for; its features scale with it
@@ -86,14 +105,16 @@ def encounter_traces(n_cap, seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_OUT)
-    ap.add_argument("--kepler-samples", type=int, default=10_000_000)
+    ap.add_argument("--kepler-samples", type=int, default=10_000_000,
+                    help="samples from the C1/C3 runs (tau above the cut)")
+    ap.add_argument("--belt-samples", type=int, default=2_000_000, help="samples from the inner belt")
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
     t0 = time.time()
     report = {}
 
-    f, t = kepler_traces(args.kepler_samples, args.seed)
+    f, t, counts = kepler_traces(args.kepler_samples, args.belt_samples, args.seed)
     mu_t, sd_t = float(np.mean(t)), float(np.std(t) + 1e-30)
     tn = ((t - mu_t) / sd_t).astype(np.float32)
     with mint.Mint(dt=0.5) as g:
@@ -104,7 +125,7 @@ def main():
     W3 = slice(4 * 32 + 32 + 32 * 32 + 32, 4 * 32 + 32 + 32 * 32 + 32 + 32)
     w[W3] *= sd_t
     w[-1] = w[-1] * sd_t + mu_t
-    report["kepler"] = dict(samples=int(len(t)), target_mean=mu_t, target_std=sd_t,
+    report["kepler"] = dict(samples=int(len(t)), samples_by_source=counts, target_mean=mu_t, target_std=sd_t,
                             loss_standardised=[float(x) for x in losses], tau_min=TRAIN_TAU_MIN)
     wk = w
 

@@ -178,3 +178,41 @@ def test_scorer_far_pair_bound_covers_the_network_and_skipping_changes_no_pick()
           f"{s0['prebucketed']} (all pairs) vs {s1['prebucketed']} (far pairs left out)", flush=True)
     assert s0["prebucketed"] == s1["prebucketed"] > 0
     assert np.array_equal(o0.tx, o1.tx) and np.array_equal(o0.tv, o1.tv)
+
+
+def test_learned_guess_on_the_sweep_has_no_newton_iteration_to_save():
+    """The Kepler network on its own target workload (the 4096-system sweep,
+    learned+batched form): every drift there, test particles and giants
+    alike, already converges in the first Newton evaluation from the
+    third-order series guess, so no warm start can take iterations away.
+    Measured on a sweep slice over two launch windows: the plain form's crude
+    guess needs about 1.75 evaluations per TP drift, the series and the
+    learned guesses exactly 1 (the learned guess's gate, tau > 0.02, never
+    opens for a TP at r >= 30 AU, so the results are bitwise those of the
+    series guess), and the oracle counts exactly one evaluation for every
+    massive body's drift too (the same algorithm, bit for bit, as the
+    ephemeris kernel's)."""
+    st = W.p9_sweep(systems=np.arange(0, 4096, 64), tp_scale=1 / 4)
+    steps = 64
+    res = {}
+    for name, form, guess in (("crude", mint.FORM_FLAT, mint.GUESS_CRUDE),
+                              ("series", mint.FORM_BATCHED, mint.GUESS_SERIES),
+                              ("learned", mint.FORM_BATCHED, mint.GUESS_LEARNED)):
+        with mint.Mint(dt=0.5, form=form, kepler_guess=guess, collect_stats=1, chunk_steps=32, block_size=256,
+                       tps_per_lane=1) as g:
+            if guess == mint.GUESS_LEARNED:
+                _weights(g)
+            g.upload(st)
+            g.step(steps)
+            s = g.stats()
+            res[name] = (s["newton_iters"] / s["kepler_solves"], g.download())
+    giants = mint.State(np.zeros(st.n_systems + 1, np.int64), st.m, st.mx, st.mv, np.zeros((0, 3)),
+                        np.zeros((0, 3)))
+    _, _, os_ = O.run(giants, steps, 0.5)
+    mass = os_["massive_newton_iters"] / os_["massive_kepler_solves"]
+    print(f"Newton evaluations per drift on the sweep: TPs crude {res['crude'][0]:.4f}, series "
+          f"{res['series'][0]:.4f}, learned {res['learned'][0]:.4f}; massive bodies (oracle) {mass:.4f}", flush=True)
+    assert res["crude"][0] > 1.5
+    assert res["series"][0] == 1.0 and res["learned"][0] == 1.0 and mass == 1.0
+    a, b = res["series"][1], res["learned"][1]
+    assert np.array_equal(a.tx, b.tx) and np.array_equal(a.tv, b.tv) and np.array_equal(a.mx, b.mx)
