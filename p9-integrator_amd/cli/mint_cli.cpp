@@ -1,17 +1,24 @@
 // mint_cli.cpp — the `mint` command line tool on top of libmint.
 //
 //   mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p]
-//        [-s checkpoint_out] [-r checkpoint_in] [-n]
+//        [-s checkpoint_out] [-r checkpoint_in] [-n] [-a] [-g gpu]
 //
 // Same flags and JSON schema as the reference driver (README.md:10-19;
 // args_parse src/simutils.cuh:138-175; sim_from_config_file
 // src/simutils.cuh:232-301), with these documented differences:
 //   -t is an exact step count (default 1, as README.md:18 says); the
 //      reference counts 1e6-step launches (src/mercurius.cu:179).
-//   every sweep in the file runs on the selected device as its own system;
-//      a sweep replaces body 0 including its mass (the reference drops the
-//      sweep mass, src/mercurius.cu:126-164, and forces mean_anomaly to 0,
-//      src/simutils.cuh:299).
+//   -d D, as in the reference, selects the device and the sweeps [4 D, 4 D + 4)
+//      of the file (SWEEPS_PER_GPU, src/constants.cuh:12; the slice of
+//      src/simutils.cuh:288-290), each its own system, and names the output
+//      files output_<D> and elements_<D>.txt; a slice past the end of the file
+//      is an error (the reference reads past its sweep array there), a short
+//      last slice runs the sweeps it has.  -a (addition): every sweep of the
+//      file instead, as one batch (the 4096-system sweep).  -g G (addition):
+//      run on device G instead of D, so that one GPU can reproduce the files
+//      of any -d.  A sweep replaces body 0 including its mass (the reference
+//      drops the sweep mass, src/mercurius.cu:126-164, and forces
+//      mean_anomaly to 0, src/simutils.cuh:299).
 //   optional keys: "dt" (default 0.8219, src/mercurius.cu:43), "n_hill",
 //      "bs_tol", "softening2", "central_mass", "form" (plain|flat|batched),
 //      "checkpoint_every" (steps between output checkpoints; default: end only),
@@ -27,6 +34,7 @@
 // Output files keep the reference formats: positions appended to output_<dev>
 // (src/simutils.cuh:186-230) and elements in elements_<dev>.txt
 // (src/mercurius.cu:210-248; src/simutils.cuh:177-184).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -62,16 +70,20 @@ Body body_from(const jsonmin::Value &v, const std::string &dflt_name) {
   return b;
 }
 
+// sweeps each device integrates (SWEEPS_PER_GPU, src/constants.cuh:12)
+constexpr int SWEEPS_PER_DEVICE = 4;
+
 struct Args {
   std::string config, output, save, resume;
   long long steps = 1;
   int device = 0;
-  bool info = false, print_positions = false, dry = false;
+  int gpu = -1;  // -g: device to run on (default: -d)
+  bool info = false, print_positions = false, dry = false, all_sweeps = false;
 };
 
 int usage() {
   std::cerr << "usage: mint -c config.json [-t steps] [-d device] [-o positions_file] [-i] [-p] "
-               "[-s checkpoint_out] [-r checkpoint_in] [-n]\n";
+               "[-s checkpoint_out] [-r checkpoint_in] [-n] [-a] [-g gpu]\n";
   return 2;
 }
 
@@ -91,8 +103,10 @@ bool parse_args(int argc, char **argv, Args &a) {
       a.print_positions = true;
     } else if (f == "-n") {
       a.dry = true;
-    } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d" || f == "-s" || f == "-r") {
-      const char *v = need(f == "-t" || f == "-d" ? "a number" : "a file");
+    } else if (f == "-a") {
+      a.all_sweeps = true;
+    } else if (f == "-c" || f == "-t" || f == "-o" || f == "-d" || f == "-s" || f == "-r" || f == "-g") {
+      const char *v = need(f == "-t" || f == "-d" || f == "-g" ? "a number" : "a file");
       if (!v) return false;
       if (f == "-c") a.config = v;
       if (f == "-s") a.save = v;
@@ -100,12 +114,13 @@ bool parse_args(int argc, char **argv, Args &a) {
       if (f == "-o") a.output = v;
       if (f == "-t") a.steps = std::atoll(v);
       if (f == "-d") a.device = std::atoi(v);
+      if (f == "-g") a.gpu = std::atoi(v);
     } else {
       std::cerr << "mint: unknown flag " << f << "\n";
       return false;
     }
   }
-  return !a.config.empty() && a.steps >= 0;
+  return !a.config.empty() && a.steps >= 0 && a.device >= 0;
 }
 
 void check(int rc, void *ctx, const char *what) {
@@ -144,6 +159,16 @@ int main(int argc, char **argv) {
     std::cerr << "mint: no bodies in " << args.config << "\n";
     return 1;
   }
+  if (!sweeps.empty() && !args.all_sweeps) {
+    const size_t first = (size_t)SWEEPS_PER_DEVICE * (size_t)args.device;
+    if (first >= sweeps.size()) {
+      std::cerr << "mint: -d " << args.device << " selects sweeps " << first << ".." << first + SWEEPS_PER_DEVICE - 1
+                << " but " << args.config << " has " << sweeps.size() << " (use -a for all of them)\n";
+      return 1;
+    }
+    const size_t last = std::min(sweeps.size(), first + SWEEPS_PER_DEVICE);
+    sweeps = std::vector<Body>(sweeps.begin() + (long)first, sweeps.begin() + (long)last);
+  }
 
   mint_config cfg;
   mint_config_default(&cfg);
@@ -154,7 +179,7 @@ int main(int argc, char **argv) {
   cfg.central_mass = cfgj.number_or("central_mass", cfg.central_mass);
   const std::string form = cfgj.string_or("form", "batched");
   cfg.form = form == "plain" ? MINT_FORM_PLAIN : form == "flat" ? MINT_FORM_FLAT : MINT_FORM_BATCHED;
-  cfg.device = args.device;
+  cfg.device = args.gpu >= 0 ? args.gpu : args.device;
   const long long ckpt = (long long)cfgj.number_or("checkpoint_every", (double)args.steps);
 
   // systems: one per sweep (sweep replaces body 0), or the bodies as given

@@ -66,8 +66,8 @@ def test_embryo_config_heaviest_first():
     assert m[0] < 0.2 / 332946 and m[-1] == pytest.approx(0.6 / 27068510, rel=1e-15)
 
 
-def _dry(path):
-    r = subprocess.run([CLI, "-c", path, "-n"], capture_output=True, text=True, timeout=120)
+def _dry(path, *flags):
+    r = subprocess.run([CLI, "-c", path, "-n", *flags], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     return r.stdout.split(": ", 1)[1]
 
@@ -82,7 +82,15 @@ def test_cli_reads_generated_configs(tmp_path):
     for name, (expect, cfg) in cases.items():
         p = tmp_path / f"{name}.json"
         p.write_text(json.dumps(cfg))
-        assert _dry(str(p)).startswith(expect), name
+        assert _dry(str(p), "-a").startswith(expect), name
+        # the reference's per-device slice: sweeps [4 d, 4 d + 4)
+        n = int(expect.split()[0])
+        assert _dry(str(p)).startswith(f"{min(n, 4)} system(s)"), name
+    last = (32 - 1) // 4
+    assert _dry(str(tmp_path / "p9.json"), "-d", str(last)).startswith("4 system(s)")
+    r = subprocess.run([CLI, "-c", str(tmp_path / "p9.json"), "-n", "-d", str(last + 1)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "selects sweeps 32..35" in r.stderr
     assert "dt 0.002 yr, n_hill 3, bs_tol 1e-11" in _dry(str(tmp_path / "embryos_1000.json"))
 
 
@@ -93,7 +101,8 @@ def test_cli_reads_the_reference_examples_like_the_generated_ones(tmp_path):
     p.write_text(json.dumps(MC.p9_config()))
     # same shape and the same total massive mass (the sweep mass ladder does not
     # depend on the shuffle)
-    assert _dry(os.path.join(REF_EXAMPLES, "p9.json")) == _dry(str(p))
+    assert _dry(os.path.join(REF_EXAMPLES, "p9.json"), "-a") == _dry(str(p), "-a")
+    assert _dry(os.path.join(REF_EXAMPLES, "p9.json")).startswith("4 system(s), 5 massive")
     e = tmp_path / "e.json"
     e.write_text(json.dumps(MC.embryo_config()))
     ref_e = _dry(os.path.join(REF_EXAMPLES, "planetary_embryos.json"))
