@@ -2,7 +2,8 @@
 counts that are not a multiple of the window (and shorter than one), one-step
 windows, a single TP, and systems without TPs between systems with them,
 with close encounters carried across window boundaries; non-finite TPs; TPs
-on unbound and near-parabolic orbits (plain form too)."""
+on unbound and near-parabolic orbits, and on close-in orbits whose drifts
+take the Stumpff functions' argument quartering (plain form too)."""
 import numpy as np
 import pytest
 
@@ -81,14 +82,52 @@ def test_non_finite_tps_are_flagged_and_isolated(form):
 def test_unbound_and_near_parabolic_tps_match_the_oracle(form):
     """TPs on hyperbolic orbits, within 1e-9 of the parabolic limit, around
     escape and slow and nearly radial: the universal-variable drift's
-    negative-alpha Stumpff branches and large-|z| quartering over 100 steps,
-    bit for bit against the oracle."""
+    negative-alpha Stumpff branches (at 20-200 AU every |z| stays below the
+    series band's 0.1; the close-in test below takes the quartering) over 100
+    steps, bit for bit against the oracle."""
     st = unbound_cloud()
     print(f"unbound cloud form={form}", flush=True)
     kw = dict(chunk_steps=32, block_size=64, tps_per_lane=1) if form == mint.FORM_BATCHED else {}
     out, status, s = run_gpu(st, 100, 0.5, form, mint.GUESS_SERIES, **kw)
     ref, rst, rs = O.run(st, 100, 0.5, guess=1)
     assert np.all(np.isfinite(ref["tx"])) and np.abs(ref["tx"]).max() > 200.0
+    assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
+    assert np.array_equal(out.mx, ref["mx"]) and np.all(status == rst)
+    assert s["newton_iters"] == rs["newton_iters"] > rs["kepler_solves"]
+
+
+def _close_in(n=256, seed=23):
+    """The giants of sweep system 0 with TPs on close-in orbits (a 0.05-1.2
+    AU, e < 0.3, i < 10 deg): at dt = 0.5 yr a drift spans up to several
+    orbits, so z = beta s^2 of the Kepler solve runs from below the series
+    band's 0.1 to beyond 1e3 (up to six quarterings of the argument)."""
+    st = W.p9_sweep(systems=[0], tp_scale=1 / 64).copy()
+    rng = np.random.default_rng([20261203, 11, seed])
+    a = np.exp(rng.uniform(np.log(0.05), np.log(1.2), n))
+    e = rng.uniform(0.0, 0.3, n)
+    inc = np.radians(rng.uniform(0.0, 10.0, n))
+    ang = rng.uniform(0.0, 2 * np.pi, (3, n))
+    st.tx, st.tv = mint.elements_to_cartesian(1.0, inc, ang[0], ang[1], ang[2], e, a)
+    st.tp_offset = np.array([0, n], np.int64)
+    return st
+
+
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_BATCHED])
+def test_close_in_orbits_take_the_stumpff_quartering_and_match_the_oracle(form):
+    st = _close_in()
+    # the first drift of every TP, from the oracle's converged root: z = beta s^2
+    z = []
+    for x, v in zip(st.tx, st.tv):
+        _, _, _, s = O.kepler_drift(x, v, 1.0, 0.5)
+        z.append((2.0 / np.linalg.norm(x) - v @ v) * s * s)
+    z = np.abs(np.array(z))
+    print(f"close-in cloud form={form}: |z| from {z.min():.3g} to {z.max():.3g}, "
+          f"{np.mean(z > 0.1):.2f} of the drifts above 0.1", flush=True)
+    assert np.mean(z > 0.1) > 0.5 and np.max(z) > 0.1 * 4 ** 5
+    kw = dict(chunk_steps=16, block_size=64, tps_per_lane=1) if form == mint.FORM_BATCHED else {}
+    out, status, s = run_gpu(st, 50, 0.5, form, mint.GUESS_SERIES, **kw)
+    ref, rst, rs = O.run(st, 50, 0.5, guess=1)
+    assert np.all(np.isfinite(ref["tx"])) and not np.any(rst & mint.ST_NEWTON)
     assert np.array_equal(out.tx, ref["tx"]) and np.array_equal(out.tv, ref["tv"])
     assert np.array_equal(out.mx, ref["mx"]) and np.all(status == rst)
     assert s["newton_iters"] == rs["newton_iters"] > rs["kepler_solves"]
