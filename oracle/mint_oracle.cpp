@@ -185,8 +185,9 @@ Rc pair_rc(const Rc &a, const Rc &b) {
 }
 
 // Sums over massive bodies run in chunks of CH bodies (index order): each
-// chunk's sum starts from zero, and the chunk sums are added in chunk order.
-// Pair weights are divided in groups of DG bodies with one division each.
+// chunk's sum starts from zero, and the chunk sums are folded by halving
+// strides (fold_chunks).  Pair weights are divided in groups of DG bodies with
+// one division each.
 constexpr int CH = 16;
 constexpr int DG = 8;
 
@@ -270,9 +271,22 @@ V3 accel_tp(const V3 &x, const std::vector<V3> &X, const Sys &s, double eps2) {
 // src/sim.cuh:327-382, over massive partners, self excluded by index).  In a
 // division group the body itself is an inert slot (numerator 0, denominator 1,
 // separation 0), which leaves every other weight and the sum unchanged.
+// The fold of chunk sums a[0..n): with P = n rounded up to a power of two and
+// a[l] = +0 for l >= n, slot l < h takes a[l] + a[l + h] for h = P/2, ..., 1;
+// the result is a[0] (one chunk: its own sum).  This is the order of a
+// wavefront's shuffle reduction over one chunk per lane.
+V3 fold_chunks(std::vector<V3> a) {
+  size_t P = 1;
+  while (P < a.size()) P <<= 1;
+  a.resize(P, V3{0.0, 0.0, 0.0});
+  for (size_t h = P / 2; h >= 1; h /= 2)
+    for (size_t l = 0; l < h; ++l) a[l] = {a[l].x + a[l + h].x, a[l].y + a[l + h].y, a[l].z + a[l + h].z};
+  return a[0];
+}
+
 V3 accel_one(int i, const std::vector<V3> &X, const Sys &s, double eps2) {
   const int nm = s.nm;
-  V3 a{0, 0, 0};
+  std::vector<V3> part;
   for (int c0 = 0; c0 < nm; c0 += CH) {
     const int c1 = std::min(c0 + CH, nm);
     V3 p{0, 0, 0};
@@ -298,13 +312,9 @@ V3 accel_one(int i, const std::vector<V3> &X, const Sys &s, double eps2) {
         p.z = std::fma(w[q], d[q].z, p.z);
       }
     }
-    if (c0 == 0) {
-      a = p;
-    } else {
-      a = {a.x + p.x, a.y + p.y, a.z + p.z};
-    }
+    part.push_back(p);
   }
-  return a;
+  return fold_chunks(part);
 }
 
 std::vector<V3> accel_massive(const std::vector<V3> &X, const Sys &s, double eps2) {
@@ -315,7 +325,7 @@ std::vector<V3> accel_massive(const std::vector<V3> &X, const Sys &s, double eps
 
 // Q = sum m_j v_j / M (main_body_kinetic, src/sim.cuh:384-406), chunked sum
 V3 momentum_q(const Sys &s, double M) {
-  V3 q{0, 0, 0};
+  std::vector<V3> part;
   for (int c0 = 0; c0 < s.nm; c0 += CH) {
     V3 p{0, 0, 0};
     for (int j = c0; j < std::min(c0 + CH, s.nm); ++j) {
@@ -323,12 +333,9 @@ V3 momentum_q(const Sys &s, double M) {
       p.y = std::fma(s.m[j], s.mb[j].v.y, p.y);
       p.z = std::fma(s.m[j], s.mb[j].v.z, p.z);
     }
-    if (c0 == 0) {
-      q = p;
-    } else {
-      q = {q.x + p.x, q.y + p.y, q.z + p.z};
-    }
+    part.push_back(p);
   }
+  const V3 q = fold_chunks(part);
   return {q.x / M, q.y / M, q.z / M};
 }
 

@@ -22,8 +22,9 @@ constexpr uint32_t ST_NAN = 1u, ST_NEWTON = 2u, ST_BS_DEPTH = 4u, ST_MASSIVE_ENC
 constexpr uint32_t ST_OVERLAP = 16u, ST_LIMIT = 32u;
 
 // Sums over massive bodies run in chunks of MB_CHUNK bodies in index order:
-// each chunk's sum starts from zero and the chunk sums are added in chunk
-// order.  Kick weights are divided in groups of MB_DIVG bodies with one
+// each chunk's sum starts from zero and the chunk sums are folded by halving
+// strides (slot l takes slot l + h for h = P/2, ..., 1 over the chunk count
+// rounded up to a power of two P, missing chunks +0; one chunk: its own sum).  Kick weights are divided in groups of MB_DIVG bodies with one
 // division per group (batched_div), the body itself an inert slot.  Systems of
 // up to 8 massive bodies are one chunk and one division group.
 constexpr int MB_CHUNK = 16;

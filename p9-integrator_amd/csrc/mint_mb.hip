@@ -28,8 +28,9 @@
 //
 // Kick sums: lane (i, chunk) accumulates the 16 partners of one chunk from
 // zero, two division groups of 8 with one division each (batched_div), the
-// body itself and bodies past N as inert slots; the chunk sums are then added
-// in chunk order.  The momentum sum uses the same chunks.
+// body itself and bodies past N as inert slots; the chunk sums are then folded
+// by halving strides (fold_chunks).  The momentum sum uses the same chunks and
+// the same fold.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +47,7 @@ struct MBShape {
   int TI;   // bodies per workgroup (open/close kernels)
   int CS;   // chunk slots per workgroup; TI * CS = MB_BLOCK; a window of CS
             // chunks of partner data is staged in LDS at a time
+  int P;    // nch rounded up to a power of two (the chunk fold's width)
 };
 
 constexpr int MB_BLOCK = 256;
@@ -61,6 +63,9 @@ MBShape mb_shape(int N) {
   while (cs < s.nch && cs < MB_MAXCS) cs <<= 1;
   s.CS = cs;
   s.TI = MB_BLOCK / cs;
+  int p = 1;
+  while (p < s.nch) p <<= 1;
+  s.P = p;  // = CS up to 64 chunks (1024 bodies), 128 beyond
   return s;
 }
 
@@ -74,35 +79,32 @@ __device__ __forceinline__ BP bp_of(const double *bp_all, int s, int N) {
   return BP{b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
 }
 
-// Left fold p[0] + p[1] + ... + p[n-1] of n three-vectors stored [n][3] in
-// LDS (loads issued eight at a time ahead of the dependent adds).
-__device__ __forceinline__ void fold3(const double *p, int n, int stride, double &x, double &y, double &z) {
-  x = p[0];
-  y = p[1];
-  z = p[2];
-  int c = 1;
-  for (; c + 8 <= n; c += 8) {
-    double tx[8], ty[8], tz[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const double *q = p + (size_t)(c + u) * stride;
-      tx[u] = q[0];
-      ty[u] = q[1];
-      tz[u] = q[2];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      x = x + tx[u];
-      y = y + ty[u];
-      z = z + tz[u];
-    }
+// Chunk sums a[0..nch) are folded by halving strides: with P = nch rounded
+// up to a power of two and a[l] = +0 for l >= nch, slot l < h takes
+// a[l] + a[l + h] for h = P/2, ..., 1, and the fold is a[0] (one chunk: its
+// own sum).  Lanes [0, G) of a lane group hold a[l] (and, for P = 128, have
+// already added a[l + 64]); G = min(P, 64) divides 64.  The fold ends in the
+// group's first lane.
+__device__ __forceinline__ void fold_lanes(double &x, double &y, double &z, int G) {
+  for (int h = G >> 1; h > 0; h >>= 1) {
+    x = x + __shfl_down(x, h, G);
+    y = y + __shfl_down(y, h, G);
+    z = z + __shfl_down(z, h, G);
   }
-  for (; c < n; ++c) {
-    const double *q = p + (size_t)c * stride;
-    x = x + q[0];
-    y = y + q[1];
-    z = z + q[2];
-  }
+}
+
+// kick_nd formed without branches: both changeover forms are evaluated and
+// selected (the same values kick_nd gives, NaN inputs included).
+__device__ __forceinline__ void kick_nd_sel(double r2, double eps2, double m, double rclo, double rcwinv,
+                                            double &num, double &den) {
+  const double r = sqrt_dist2(r2);
+  const double D = (r2 + eps2) * r;
+  const double yv = (r - rclo) * rcwinv;
+  const double y2 = yv * yv;
+  const double nb = m * y2, db = D * fma(2.0, y2, fma(-2.0, yv, 1.0));
+  const bool band = yv > 0.0 && yv < 1.0;
+  num = yv <= 0.0 ? 0.0 : (band ? nb : m);
+  den = band ? db : D;
 }
 
 // Staged partner windows in LDS carry one double of padding after each
@@ -119,42 +121,89 @@ __device__ __forceinline__ int lds1(int k) { return k + (k >> 4); }      // scal
 // when the values go to LDS, once the momentum is known.
 constexpr int MB_PF = MB_MAXCS * MB_CHUNK / MB_BLOCK;
 
-// Q = (sum_j m_j v_j)/M by the whole workgroup: chunk sums (from zero, index
-// order) in part[], then added in chunk order by one thread.
-__device__ void wg_momentum(const double *__restrict__ V, const double *__restrict__ m, int N,
-                            int nch, double M, double *part, double *Q) {
-  for (int c = threadIdx.x; c < nch; c += blockDim.x) {
-    const int k0 = c * MB_CHUNK;
-    double mm[MB_CHUNK], vx[MB_CHUNK], vy[MB_CHUNK], vz[MB_CHUNK];
+// Q = (sum_j m_j v_j)/M by the whole workgroup.  Masses and velocities pass
+// through LDS a window of WB bodies at a time (sv in the lds3 layout, sm in
+// the lds1 layout; the first window comes from the caller's registers pm, pv,
+// loaded by body index k = threadIdx.x + u MB_BLOCK, so every load is
+// coalesced); lane c of wavefront 0 forms the window's chunk c sum (from zero,
+// index order), and the chunk sums are folded (fold_lanes).  At most two
+// windows: N <= 2048 = 2 WB.  Qs[3] receives Q; the caller's next write to sv
+// or sm follows the barrier this ends with.
+__device__ void wg_momentum(const double *__restrict__ V, const double *__restrict__ m, const MBShape &sh,
+                            double M, const double (&pm)[MB_PF], const double (&pv)[MB_PF][3], double *sv,
+                            double *sm, double *Qs) {
+  const int t = (int)threadIdx.x, N = sh.N, WB = sh.CS * MB_CHUNK;
+  double c0x = 0.0, c0y = 0.0, c0z = 0.0, c1x = 0.0, c1y = 0.0, c1z = 0.0;
+  for (int b0 = 0, pass = 0; b0 < N; b0 += WB, ++pass) {
+    const int nw = min(WB, N - b0);
+    if (pass > 0) __syncthreads();  // wavefront 0 has read the previous window
+    if (pass == 0) {
 #pragma unroll
-    for (int q = 0; q < MB_CHUNK; ++q) {
-      const int k = k0 + q;
-      const bool in = k < N;
-      mm[q] = in ? m[k] : 0.0;
-      vx[q] = in ? V[3 * k + 0] : 0.0;
-      vy[q] = in ? V[3 * k + 1] : 0.0;
-      vz[q] = in ? V[3 * k + 2] : 0.0;
-    }
-    double px = 0.0, py = 0.0, pz = 0.0;
-#pragma unroll
-    for (int q = 0; q < MB_CHUNK; ++q) {
-      if (k0 + q < N) {
-        px = fma(mm[q], vx[q], px);
-        py = fma(mm[q], vy[q], py);
-        pz = fma(mm[q], vz[q], pz);
+      for (int u = 0; u < MB_PF; ++u) {
+        const int k = t + u * MB_BLOCK;
+        if (k < nw) {
+          sv[lds3(k) + 0] = pv[u][0];
+          sv[lds3(k) + 1] = pv[u][1];
+          sv[lds3(k) + 2] = pv[u][2];
+          sm[lds1(k)] = pm[u];
+        }
+      }
+    } else {
+      for (int k = t; k < nw; k += MB_BLOCK) {
+        const int kk = b0 + k;
+        sv[lds3(k) + 0] = V[3 * kk + 0];
+        sv[lds3(k) + 1] = V[3 * kk + 1];
+        sv[lds3(k) + 2] = V[3 * kk + 2];
+        sm[lds1(k)] = m[kk];
       }
     }
-    part[3 * c + 0] = px;
-    part[3 * c + 1] = py;
-    part[3 * c + 2] = pz;
+    __syncthreads();
+    if (t * MB_CHUNK < nw) {  // t < CS <= 64: wavefront 0
+      double px = 0.0, py = 0.0, pz = 0.0;
+#pragma unroll
+      for (int h = 0; h < MB_CHUNK; h += MB_DIVG) {  // eight bodies' loads at a time
+        // slots past the window (not staged) are read and their terms
+        // selected away, without branches
+        double mm[MB_DIVG], vx[MB_DIVG], vy[MB_DIVG], vz[MB_DIVG];
+#pragma unroll
+        for (int q = 0; q < MB_DIVG; ++q) {
+          const int k = t * MB_CHUNK + h + q;
+          mm[q] = sm[lds1(k)];
+          vx[q] = sv[lds3(k) + 0];
+          vy[q] = sv[lds3(k) + 1];
+          vz[q] = sv[lds3(k) + 2];
+        }
+#pragma unroll
+        for (int q = 0; q < MB_DIVG; ++q) {
+          const bool in = t * MB_CHUNK + h + q < nw;
+          px = in ? fma(mm[q], vx[q], px) : px;
+          py = in ? fma(mm[q], vy[q], py) : py;
+          pz = in ? fma(mm[q], vz[q], pz) : pz;
+        }
+      }
+      if (pass == 0) {
+        c0x = px;
+        c0y = py;
+        c0z = pz;
+      } else {
+        c1x = px;
+        c1y = py;
+        c1z = pz;
+      }
+    }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double qx, qy, qz;
-    fold3(part, nch, 3, qx, qy, qz);
-    Q[0] = qx / M;
-    Q[1] = qy / M;
-    Q[2] = qz / M;
+  if (t < 64) {
+    if (sh.P > 64) {  // first level of the fold: chunk l + 64 (or +0)
+      c0x = c0x + c1x;
+      c0y = c0y + c1y;
+      c0z = c0z + c1z;
+    }
+    fold_lanes(c0x, c0y, c0z, sh.P < 64 ? sh.P : 64);
+    if (t == 0) {
+      Qs[0] = c0x / M;
+      Qs[1] = c0y / M;
+      Qs[2] = c0z / M;
+    }
   }
   __syncthreads();
 }
@@ -182,9 +231,6 @@ __global__ void __launch_bounds__(MB_BLOCK)
   const double *VB = vb_all + (size_t)s * N * 3;
   const int WB = sh.CS * MB_CHUNK;                   // bodies per staged window
   double *part = lds;                                // [nch][TI][3]
-  // momentum chunk sums [nch][3]: read before the barrier that opens the first
-  // window, so they share part's space (part is written after it)
-  double *qpart = part;
   double *Qs = part + (size_t)sh.nch * sh.TI * 3;    // [3]
   double *sx = Qs + 3;                               // partner positions, lds3 layout
   double *sm = sx + 3 * WB + sh.CS;                  // per-body scalars, lds1 layout
@@ -208,7 +254,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
       vz = V[3 * i + 2];
     }
   }
-  double pf[MB_PF][6];
+  double pf[MB_PF][6], pm[MB_PF], pv[MB_PF][3];
 #pragma unroll
   for (int u = 0; u < MB_PF; ++u) {
     const int k = (int)threadIdx.x + u * MB_BLOCK;
@@ -219,11 +265,19 @@ __global__ void __launch_bounds__(MB_BLOCK)
       pf[u][3] = bp.m[k];
       pf[u][4] = bp.lo[k];
       pf[u][5] = bp.winv[k];
+      pm[u] = pf[u][3];
+      if (!INIT) {
+        pv[u][0] = V[3 * k + 0];
+        pv[u][1] = V[3 * k + 1];
+        pv[u][2] = V[3 * k + 2];
+      }
     }
   }
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (!INIT) {
-    wg_momentum(VB, bp.m, N, sh.nch, M, qpart, Qs);
+    // the velocities pass through the partner window's LDS (sx, sm), which
+    // the window loop below fills after the barrier wg_momentum ends with
+    wg_momentum(VB, bp.m, sh, M, pm, pv, sx, sm, Qs);
     q0 = Qs[0];
     q1 = Qs[1];
     q2 = Qs[2];
@@ -269,21 +323,22 @@ __global__ void __launch_bounds__(MB_BLOCK)
         double dx[MB_DIVG], dy[MB_DIVG], dz[MB_DIVG], num[MB_DIVG], den[MB_DIVG], w[MB_DIVG];
 #pragma unroll
         for (int q = 0; q < MB_DIVG; ++q) {
+          // every slot is formed and the inert ones (the body itself, slots
+          // past N, whose window entries are not staged) are selected away,
+          // so that the eight pairs' chains interleave without branches
           const int k = c * MB_CHUNK + hg * MB_DIVG + q;
           const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
-          dx[q] = 0.0;
-          dy[q] = 0.0;
-          dz[q] = 0.0;
-          num[q] = 0.0;
-          den[q] = 1.0;
-          if (k < N && k != i) {
-            dx[q] = sx[k3 + 0] - xi;
-            dy[q] = sx[k3 + 1] - yi;
-            dz[q] = sx[k3 + 2] - zi;
-            const double r2 = dot3(dx[q], dy[q], dz[q], dx[q], dy[q], dz[q]);
-            const double lok = slo[k1], wvk = swv[k1];
-            kick_nd(r2, sc.eps2, sm[k1], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, num[q], den[q]);
-          }
+          const bool live = k < N && k != i;
+          const double ex = sx[k3 + 0] - xi, ey = sx[k3 + 1] - yi, ez = sx[k3 + 2] - zi;
+          const double lok = slo[k1], wvk = swv[k1];
+          double nq, dq;
+          kick_nd_sel(dot3(ex, ey, ez, ex, ey, ez), sc.eps2, sm[k1], loi > lok ? loi : lok,
+                      wvi < wvk ? wvi : wvk, nq, dq);
+          dx[q] = live ? ex : 0.0;
+          dy[q] = live ? ey : 0.0;
+          dz[q] = live ? ez : 0.0;
+          num[q] = live ? nq : 0.0;
+          den[q] = live ? dq : 1.0;
         }
         batched_div<MB_DIVG>(num, den, w);
 #pragma unroll
@@ -300,9 +355,35 @@ __global__ void __launch_bounds__(MB_BLOCK)
     pp[2] = pz;
   }
   __syncthreads();
+  {
+    // fold of body b's chunk sums by lanes [b G, (b + 1) G); TI G = MB_BLOCK
+    // (G = min(P, 64) = CS); the folds go to acc[TI][3] over the spent window
+    const int G = sh.P < 64 ? sh.P : 64;
+    const int b = (int)threadIdx.x / G, l = (int)threadIdx.x % G;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (l < sh.nch) {
+      const double *pp = part + ((size_t)l * sh.TI + b) * 3;
+      ax = pp[0];
+      ay = pp[1];
+      az = pp[2];
+    }
+    if (sh.P > 64) {  // first level of the fold: chunk l + 64 (or +0)
+      const bool in = l + 64 < sh.nch;
+      const double *pp = part + ((size_t)(in ? l + 64 : 0) * sh.TI + b) * 3;
+      ax = ax + (in ? pp[0] : 0.0);
+      ay = ay + (in ? pp[1] : 0.0);
+      az = az + (in ? pp[2] : 0.0);
+    }
+    fold_lanes(ax, ay, az, G);
+    if (l == 0) {
+      sx[3 * b + 0] = ax;
+      sx[3 * b + 1] = ay;
+      sx[3 * b + 2] = az;
+    }
+  }
+  __syncthreads();
   if (cs == 0 && act) {
-    double ax, ay, az;
-    fold3(part + 3 * ti, sh.nch, 3 * sh.TI, ax, ay, az);
+    const double ax = sx[3 * ti + 0], ay = sx[3 * ti + 1], az = sx[3 * ti + 2];
     vx = fma(sc.hh, ax, vx);
     vy = fma(sc.hh, ay, vy);
     vz = fma(sc.hh, az, vz);
@@ -467,8 +548,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
   int *edge = edge_all + (size_t)s * MB_EMAX * 2;
   unsigned int *ctl = ctl_all + 4 * s;
   const int WB = sh.CS * MB_CHUNK;
-  double *qpart = lds;
-  double *Qs = lds + 3 * sh.nch;
+  double *Qs = lds;
   double *sx = Qs + 3;                 // partner positions after jump 1, lds3 layout
   double *src2 = sx + 3 * WB + sh.CS;  // per-body scalars, lds1 layout
   double *srad = src2 + WB + sh.CS;
@@ -485,7 +565,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
     rc2i = bp.rc2[i];
     radi = bp.rad[i];
   }
-  double pf[MB_PF][5];
+  double pf[MB_PF][5], pm[MB_PF], pv[MB_PF][3];
 #pragma unroll
   for (int u = 0; u < MB_PF; ++u) {
     const int k = (int)threadIdx.x + u * MB_BLOCK;
@@ -495,9 +575,15 @@ __global__ void __launch_bounds__(MB_BLOCK)
       pf[u][2] = XA[3 * k + 2];
       pf[u][3] = bp.rc2[k];
       pf[u][4] = bp.rad[k];
+      pm[u] = bp.m[k];
+      pv[u][0] = VA[3 * k + 0];
+      pv[u][1] = VA[3 * k + 1];
+      pv[u][2] = VA[3 * k + 2];
     }
   }
-  wg_momentum(VA, bp.m, N, sh.nch, M, qpart, Qs);
+  // the velocities pass through the partner window's LDS (sx, src2), which
+  // the window loop below fills after the barrier wg_momentum ends with
+  wg_momentum(VA, bp.m, sh, M, pm, pv, sx, src2, Qs);
   const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
   if (act) {
     xi = fma(sc.hh, q0, xi);
@@ -818,7 +904,7 @@ size_t close_lds(const MBShape &sh) {
 
 size_t open_lds(const MBShape &sh) {
   const size_t WB = (size_t)sh.CS * MB_CHUNK;
-  return ((size_t)sh.nch * 3 + 3 + (3 * WB + sh.CS) + 2 * (WB + sh.CS)) * sizeof(double);
+  return ((size_t)3 + (3 * WB + sh.CS) + 2 * (WB + sh.CS)) * sizeof(double);
 }
 
 size_t drift_lds(const MBShape &) { return ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int); }

@@ -82,6 +82,34 @@ def test_accel_small_set_is_one_chunk_with_inert_self():
         assert np.allclose(a[i], acc, rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("n", [40, 100, 1500])
+def test_accel_folds_chunk_sums_by_halving_strides(n):
+    # the kick sum of a body is its 16-body chunk sums folded as a wavefront
+    # reduction folds them: pad to a power of two with +0, then slot l takes
+    # slot l + h for h = P/2, ..., 1.  Each chunk's own sum is isolated by
+    # giving every other chunk zero mass (its terms are then +0 exactly).
+    m, rc, x = _cloud(n, 11)
+    a = O.accel_massive(m, rc, x)
+    nch = (n + 15) // 16
+    parts = []
+    for c in range(nch):
+        mm = np.zeros(n)
+        mm[16 * c:16 * c + 16] = m[16 * c:16 * c + 16]
+        parts.append(O.accel_massive(mm, rc, x))
+    P = 1 << (nch - 1).bit_length()
+    slots = parts + [np.zeros_like(a)] * (P - nch)
+    h = P // 2
+    while h >= 1:
+        slots = [slots[l] + slots[l + h] for l in range(h)]
+        h //= 2
+    assert np.array_equal(a, slots[0])
+    left = parts[0]
+    for p in parts[1:]:
+        left = left + p
+    assert not np.array_equal(a, left)  # the order is visible: a left fold differs
+    assert np.allclose(a, left, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_groups_are_connected_components(seed):
     rng = np.random.default_rng(seed)
