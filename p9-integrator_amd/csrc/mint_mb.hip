@@ -85,11 +85,50 @@ __device__ __forceinline__ BP bp_of(const double *bp_all, int s, int N) {
 // own sum).  Lanes [0, G) of a lane group hold a[l] (and, for P = 128, have
 // already added a[l + 64]); G = min(P, 64) divides 64.  The fold ends in the
 // group's first lane.
+//
+// DPP moves (gfx9 data-parallel primitives; all lanes of the wavefront
+// active): dpp64<C> returns lane l's view of x through the DPP control C,
+// 0 where the source lane is outside the row.  Controls used: 0x101..0x108
+// row_shl:n (lane l reads l + n within its 16-lane row), 0x111..0x118
+// row_shr:n, 0x142 row_bcast:15, 0x143 row_bcast:31, 0xB1 / 0x4E quad_perm
+// [1,0,3,2] / [2,3,0,1], 0x141 row_half_mirror, 0x140 row_mirror.
+template <int C>
+__device__ __forceinline__ double dpp64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), C, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), C, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ void fold_lanes(double &x, double &y, double &z, int G) {
-  for (int h = G >> 1; h > 0; h >>= 1) {
-    x = x + __shfl_down(x, h, G);
-    y = y + __shfl_down(y, h, G);
-    z = z + __shfl_down(z, h, G);
+  if (G > 32) {
+    x = x + __shfl_down(x, 32, 64);
+    y = y + __shfl_down(y, 32, 64);
+    z = z + __shfl_down(z, 32, 64);
+  }
+  if (G > 16) {
+    x = x + __shfl_down(x, 16, 64);
+    y = y + __shfl_down(y, 16, 64);
+    z = z + __shfl_down(z, 16, 64);
+  }
+  if (G > 8) {
+    x = x + dpp64<0x108>(x);
+    y = y + dpp64<0x108>(y);
+    z = z + dpp64<0x108>(z);
+  }
+  if (G > 4) {
+    x = x + dpp64<0x104>(x);
+    y = y + dpp64<0x104>(y);
+    z = z + dpp64<0x104>(z);
+  }
+  if (G > 2) {
+    x = x + dpp64<0x102>(x);
+    y = y + dpp64<0x102>(y);
+    z = z + dpp64<0x102>(z);
+  }
+  if (G > 1) {
+    x = x + dpp64<0x101>(x);
+    y = y + dpp64<0x101>(y);
+    z = z + dpp64<0x101>(z);
   }
 }
 
@@ -420,12 +459,15 @@ __global__ void __launch_bounds__(MB_BLOCK)
 // gm[F] (body indices); returns G, and sets big when a group has more than
 // MB_GMAX members.  LDS (ints): bm[64], pre[64], and fl, L, ce, cnt, gst, gm
 // of 2 MB_EMAX each (gst one more).
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
+__device__ __forceinline__ int wave_incl_scan(int v, int) {
+  // inclusive scans within 16-lane rows (row_shr 1, 2, 4, 8, zero fill),
+  // then each row's total added to the rows after it (row_bcast 15, 31)
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
   return v;
 }
 
@@ -663,12 +705,48 @@ __global__ void __launch_bounds__(MB_BLOCK)
 // counts); the latency of an attempt drops from 2 + 4 + ... + N_k substeps to
 // N_k while k < R.  Once the halving budget is spent the rest of the interval
 // is taken in one step and flagged, as in the single-lane group BS.
+//
+// GM > 0 (GM = GP - 1, row groups of 2 or 4 lanes): the partners of member a
+// are taken in GM slots, slot s holding member s (s < a) or s + 1, so in
+// member order without the member itself; their masses and pair changeover
+// fields are read once per drift, every slot is formed without branches and
+// slots past the group are selected away, so that the partners' chains and
+// the Sun term interleave.  GM = 0: any group size, one member after the
+// other.  Same values either way.
+__device__ __forceinline__ double close_weight_sel(double r2, double eps2, double m, double rclo, double rcwinv) {
+  const double r = sqrt_dist2(r2);
+  const double yv = (r - rclo) * rcwinv;
+  const double D = (r2 + eps2) * r;
+  const double om = 1.0 - yv;
+  const double nb = m * (om * om), db = D * fma(2.0, yv * yv, fma(-2.0, yv, 1.0));
+  const bool band = yv > 0.0;
+  const double w = div_normal(band ? nb : m, band ? db : D);
+  return yv >= 1.0 ? 0.0 : w;
+}
+
+template <int GM>
 __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, double mu, double eps2,
                                   double tol, double mj, double loj, double wvj, unsigned int &nrhs) {
   const int lane = (int)(threadIdx.x & 63u);
   const int row = lane / GP, base = row * GP;
   const int R = 64 / GP < KMAX ? 64 / GP : KMAX;
   const bool act = a < gs;
+  constexpr int NS = GM > 0 ? GM : 1;
+  int psrc[NS];
+  bool plive[NS];
+  double pmb[NS], plo[NS], pwv[NS];
+  if constexpr (GM > 0) {
+#pragma unroll
+    for (int q = 0; q < GM; ++q) {
+      const int b = q < a ? q : q + 1;
+      psrc[q] = base + b;
+      plive[q] = b < gs;
+      const double lob = __shfl(loj, psrc[q], 64), wvb = __shfl(wvj, psrc[q], 64);
+      pmb[q] = __shfl(mj, psrc[q], 64);
+      plo[q] = loj > lob ? loj : lob;
+      pwv[q] = wvj < wvb ? wvj : wvb;
+    }
+  }
   auto rhs = [&](const double *z, double *f) {
     const double x = z[0], yy = z[1], zz = z[2];
     f[0] = z[3];
@@ -678,17 +756,30 @@ __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, do
     const double r = sqrt_dist2(r2);
     const double w0 = div_normal(mu, r2 * r);
     double ax = -(w0 * x), ay = -(w0 * yy), az = -(w0 * zz);
-    for (int b = 0; b < gs; ++b) {
-      const int src = base + b;
-      const double xb = __shfl(x, src, 64), yb = __shfl(yy, src, 64), zb = __shfl(zz, src, 64);
-      const double mb = __shfl(mj, src, 64), lob = __shfl(loj, src, 64), wvb = __shfl(wvj, src, 64);
-      if (b == a) continue;
-      const double dx = xb - x, dy = yb - yy, dz = zb - zz;
-      const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double w = close_weight(d2, eps2, mb, loj > lob ? loj : lob, wvj < wvb ? wvj : wvb);
-      ax = fma(w, dx, ax);
-      ay = fma(w, dy, ay);
-      az = fma(w, dz, az);
+    if constexpr (GM > 0) {
+#pragma unroll
+      for (int q = 0; q < GM; ++q) {
+        const double xb = __shfl(x, psrc[q], 64), yb = __shfl(yy, psrc[q], 64), zb = __shfl(zz, psrc[q], 64);
+        const double dx = xb - x, dy = yb - yy, dz = zb - zz;
+        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+        const double w = close_weight_sel(d2, eps2, pmb[q], plo[q], pwv[q]);
+        ax = plive[q] ? fma(w, dx, ax) : ax;
+        ay = plive[q] ? fma(w, dy, ay) : ay;
+        az = plive[q] ? fma(w, dz, az) : az;
+      }
+    } else {
+      for (int b = 0; b < gs; ++b) {
+        const int src = base + b;
+        const double xb = __shfl(x, src, 64), yb = __shfl(yy, src, 64), zb = __shfl(zz, src, 64);
+        const double mb = __shfl(mj, src, 64), lob = __shfl(loj, src, 64), wvb = __shfl(wvj, src, 64);
+        if (b == a) continue;
+        const double dx = xb - x, dy = yb - yy, dz = zb - zz;
+        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+        const double w = close_weight(d2, eps2, mb, loj > lob ? loj : lob, wvj < wvb ? wvj : wvb);
+        ax = fma(w, dx, ax);
+        ay = fma(w, dy, ay);
+        az = fma(w, dz, az);
+      }
     }
     f[3] = act ? ax : 0.0;
     f[4] = act ? ay : 0.0;
@@ -769,11 +860,20 @@ __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, do
             err = e > err ? e : err;
           }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double o = __shfl_xor(err, off, 64);
-          err = o > err ? o : err;
-        }
+        // the largest error over the wavefront (exact in any order): within
+        // rows by DPP (pairs, quads, half rows), then across rows
+        double o = dpp64<0xB1>(err);
+        err = o > err ? o : err;
+        o = dpp64<0x4E>(err);
+        err = o > err ? o : err;
+        o = dpp64<0x141>(err);
+        err = o > err ? o : err;
+        o = dpp64<0x140>(err);
+        err = o > err ? o : err;
+        o = __shfl_xor(err, 16, 64);
+        err = o > err ? o : err;
+        o = __shfl_xor(err, 32, 64);
+        err = o > err ? o : err;
         if (err <= tol) {
           conv = true;
           break;
@@ -884,7 +984,12 @@ __global__ void __launch_bounds__(64)
       loj = bp.lo[j];
       wvj = bp.winv[j];
     }
-    st |= bs_group_wave(y, a, GP, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
+    if (GP == 2)
+      st |= bs_group_wave<1>(y, a, GP, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
+    else if (GP == 4)
+      st |= bs_group_wave<3>(y, a, GP, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
+    else
+      st |= bs_group_wave<0>(y, a, GP, gs, sc.h, sc.mu, sc.eps2, sc.tol, mj, loj, wvj, nrhs);
     if (j >= 0 && lane < GP)  // every row group ends with the same state; the first writes it
       for (int c = 0; c < 3; ++c) {
         XB[3 * j + c] = y[c];
