@@ -528,6 +528,26 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
     __syncthreads();
     if (!__any(changed)) break;
   }
+  if (F <= 64) {
+    // one compact body per lane: the groups in root order by ballots (a
+    // group of at most 64 members is never too big)
+    const int Lc = lane < F ? L[lane] : -1;
+    unsigned long long roots = __ballot(lane < F && Lc == lane);
+    int start = 0, g = 0;
+    while (roots) {
+      const int r = __ffsll((long long)roots) - 1;
+      roots &= roots - 1ull;
+      const unsigned long long mem = __ballot(Lc == r);
+      if (Lc == r) gm[start + __popcll(mem & ((1ull << lane) - 1ull))] = fl[lane];
+      if (lane == 0) gst[g] = start;
+      start += __popcll(mem);
+      ++g;
+    }
+    if (lane == 0) gst[g] = F;
+    big = false;
+    __syncthreads();
+    return g;
+  }
   for (int c = lane; c < F; c += 64) atomicAdd(&cnt[L[c]], 1);
   __syncthreads();
   // scan over the compact bodies of (root, size at root), PER a lane
