@@ -616,13 +616,17 @@ __device__ __forceinline__ float scorer_rc(double rc2) { return __builtin_amdgcn
 // closest approach over the window at least 8 cut r_c).  Sufficient, without
 // a square root: r^2 >= 2 (A^2 + (|u| T)^2) implies r >= A + |u| T, and
 // |d + u t| >= |d| - |u| T over the window; A = 8 cut r_c and every term carry
-// a margin of 1e-3 over the FP32 roundings of the features.  Conservative: a
-// pair it does not rule out is checked on its formed features.
-__device__ __forceinline__ bool scorer_far(double x, double y, double z, double vx, double vy, double vz,
-                                           double X, double Y, double Z, double VX, double VY, double VZ,
-                                           float rcf, float Tf, float cut) {
-  const float dx = (float)(X - x), dy = (float)(Y - y), dz = (float)(Z - z);
-  const float ux = (float)(VX - vx), uy = (float)(VY - vy), uz = (float)(VZ - vz);
+// a margin of 1e-3 over the FP32 roundings.  The differences are taken in
+// FP32 from FP32 positions and velocities (absolute error about 2^-24 of the
+// positions' size, inside the margin wherever r reaches 8 cut r_c of a
+// planet).  A pair it does not rule out is checked on its formed features;
+// the scorer only decides which TPs start the window in the queue, so the
+// test changes no result either way.
+__device__ __forceinline__ bool scorer_far(float x, float y, float z, float vx, float vy, float vz, float X,
+                                           float Y, float Z, float VX, float VY, float VZ, float rcf, float Tf,
+                                           float cut) {
+  const float dx = X - x, dy = Y - y, dz = Z - z;
+  const float ux = VX - vx, uy = VY - vy, uz = VZ - vz;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
   const float A = (8.0f * 1.001f) * cut * rcf, uT2 = (1.001f * 1.001f) * (Tf * Tf);
@@ -829,22 +833,37 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     for (int j = 0; j < NM; ++j) sh_logit[t][j] = -INFINITY;
     float f[NM][SC_IN];
     unsigned int want = 0u;
-    if (i < n && !go) {
-      s = tp_sys[i];
-      const double *prm = prm_all + (size_t)s * prm_size(NM);
-      const double *bx = wx + (size_t)s * 3 * NM, *bv = wv + (size_t)s * 3 * NM;
+    const bool mine = i < n && !go;
+    // the pairs of this lane's TP against system sys's bodies
+    auto pairs = [&](int sys) {
+      const double *prm = prm_all + (size_t)sys * prm_size(NM);
+      const double *bx = wx + (size_t)sys * 3 * NM, *bv = wv + (size_t)sys * 3 * NM;
       const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
+      const float xf = (float)x, yf = (float)y, zf = (float)z, vxf = (float)vx, vyf = (float)vy, vzf = (float)vz;
       const double T = sc.h * (double)C;
 #pragma unroll
       for (int j = 0; j < NM; ++j) {
         const float rcf = scorer_rc(prm[P_RC2(NM) + j]);
-        if (cut <= 8.0f && scorer_far(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
-                                      bv[3 * j + 1], bv[3 * j + 2], rcf, (float)T, cut))
+        if (cut <= 8.0f && scorer_far(xf, yf, zf, vxf, vyf, vzf, (float)bx[3 * j], (float)bx[3 * j + 1],
+                                      (float)bx[3 * j + 2], (float)bv[3 * j], (float)bv[3 * j + 1],
+                                      (float)bv[3 * j + 2], rcf, (float)T, cut))
           continue;
         scorer_features(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
                         bv[3 * j + 1], bv[3 * j + 2], rcf, T, f[j]);
         if (!(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
       }
+    };
+    // TPs are stored by system, so a wavefront's lanes nearly always share
+    // one: then the system's bodies come through scalar loads, once
+    if (mine) s = tp_sys[i];
+    const unsigned long long mm = __ballot(mine);
+    const int lead = mm ? __ffsll((long long)mm) - 1 : 0;
+    const int s0 = __builtin_amdgcn_readfirstlane(__shfl(s, lead, 64));
+    const bool one = mm != 0ull && __ballot(mine && s != s0) == 0ull;
+    if (one) {
+      if (mine) pairs(s0);
+    } else if (mine) {
+      pairs(s);
     }
     // slots: a wavefront prefix sum of the lanes' counts, one LDS add per wavefront
     const unsigned int cnt = (unsigned int)__builtin_popcount(want);
