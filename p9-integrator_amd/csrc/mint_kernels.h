@@ -1060,20 +1060,20 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
 
 static __global__ void copy_u32_kernel(unsigned int *dst, const unsigned int *src) { *dst = *src; }
 
-// Window-start massive state of a call's first launch window (the ephemeris
-// writes it for every later window).
-static __global__ void copy_f64_kernel(double *__restrict__ dx, double *__restrict__ dv, const double *__restrict__ sx,
-                                       const double *__restrict__ sv, long long n) {
+// Start of a launch window: clears the window's queue counters ctl[4] and
+// system flags need[S]; for a call's first window (n > 0) also copies the
+// window-start massive state (the ephemeris writes it for every later window).
+static __global__ void window_start_kernel(unsigned int *__restrict__ ctl, unsigned int *__restrict__ need, int S,
+                                           double *__restrict__ dx, double *__restrict__ dv,
+                                           const double *__restrict__ sx, const double *__restrict__ sv,
+                                           long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 4) ctl[i] = 0u;
+  if (i < S) need[i] = 0u;
   if (i < n) {
     dx[i] = sx[i];
     dv[i] = sv[i];
   }
-}
-
-static __global__ void zero_u32_kernel(unsigned int *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
 }
 
 }  // namespace mint

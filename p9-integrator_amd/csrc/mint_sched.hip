@@ -157,10 +157,10 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     return MINT_OK;
   };
   // the first window's start state for its queue head (later windows' come
-  // from the ephemeris of the window before)
+  // from the ephemeris of the window before), with its queue reset
   const long long nb = (long long)S * c->NM * 3;
-  copy_f64_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, sm>>>(c->d_snap[0], c->d_snap[0] + nb, c->d_mx, c->d_mv,
-                                                                nb);
+  window_start_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, sm>>>(ctl[0], need[0], S, c->d_snap[0],
+                                                                    c->d_snap[0] + nb, c->d_mx, c->d_mv, nb);
   if ((rc = c->hip(hipGetLastError(), "snapshot launch"))) return rc;
   if ((rc = record(c->ev_side, 0, sm)) || (rc = wait(sp, c->ev_side, 0))) return rc;
   for (int w = 0; w < W; ++w) {
@@ -171,10 +171,12 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     if ((rc = window_eph(c, sp, C, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
     if ((rc = record(c->ev_eph, w, sp))) return rc;
     // queue head of window w, from the window-start snapshot: beside the
-    // window's ephemeris
-    zero_u32_kernel<<<1, 64, 0, sm>>>(ctl[p], 4);
-    zero_u32_kernel<<<(S + 255) / 256, 256, 0, sm>>>(need[p], S);
-    if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
+    // window's ephemeris (window 0's queue was reset with its snapshot)
+    if (w > 0) {
+      window_start_kernel<<<(unsigned)((S + 255) / 256), 256, 0, sm>>>(ctl[p], need[p], S, nullptr, nullptr,
+                                                                       nullptr, nullptr, 0);
+      if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
+    }
     if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, c->d_snap[B], c->d_snap[B] + nb, C, sc, scw,
                                         c->score_thresh, c->scorer_skip_on() ? c->sc_cut : INFINITY, c->d_carry, c->d_pre, q[p], ctl[p], need[p],
                                         slist_head[p], stats),
