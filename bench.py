@@ -42,8 +42,10 @@ N_M = 5
 # (calibrated by a 1 GiB copy: FETCH x2.0, WRITE x1.0), per launch shape
 # (block, TPs per lane, window); profiles/tp_main_kernel_p9sweep4096.md
 # and window length of the launch: {(block, TPs per lane, steps per launch): (bytes, source)}
-MEASURED_TRAFFIC = {(256, 1, 128): (959.4e6, "profiles/tp_main_kernel_p9sweep4096.md (256 steps, 2 launches)"),
-                    (256, 1, 20): (783.8e6, "profiles/tp_main_kernel_p9sweep4096_20steps.md (20-step calls)")}
+MEASURED_TRAFFIC = {(256, 1, 128): (1038.3e6, "profiles/tp_main_kernel_p9sweep4096.md (256 steps, 2 launches)"),
+                    (256, 1, 20): (783.8e6, "profiles/tp_main_kernel_p9sweep4096_20steps.md (20-step calls)"),
+                    (512, 1, 128): (1020.9e6, "profiles/tp_main_kernel_p9sweep4096_block512.md (256 steps, 2 launches)"),
+                    (512, 1, 20): (767.3e6, "profiles/tp_main_kernel_p9sweep4096_block512.md (20-step calls)")}
 PLAIN_BYTES_PER_TP_STEP = 96  # SoA6 state read and written once per step
 
 
