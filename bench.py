@@ -50,11 +50,12 @@ PLAIN_BYTES_PER_TP_STEP = 96  # SoA6 state read and written once per step
 
 
 def flops_per_tp_step(nbar: float, nm: int = N_M) -> float:
-    """Algorithmic FLOP per TP-step: two changeover half-kicks (28 per pair
-    each), two velocity updates (12), two jumps (12), the exact encounter check
-    (9 per body), and the universal-variable drift (60 + 40 per Newton
-    iteration); add/mul/div/sqrt = 1, fma = 2."""
-    return 2 * nm * 28 + 12 + 12 + 9 * nm + 60 + 40 * nbar
+    """Algorithmic FLOP per TP-step, as include/mint_flops.h defines it: two
+    changeover half-kicks (28 per pair each), two velocity updates (12), two
+    jumps (12), the exact encounter check (9 per body), and the
+    universal-variable drift (60 + 40 per Newton iteration); add/mul/div/sqrt
+    = 1, fma = 2.  The library computes it, so the bench and the header agree."""
+    return mint.flops_per_tp_step(nbar, nm)
 
 
 def log(msg):
@@ -244,9 +245,9 @@ EMBRYO_METRIC = "massive body-steps/s, 1000-embryo system (all massive, O(N^2) k
 
 def flops_per_kick_pass(n: int) -> float:
     """Algorithmic FLOP of one kick-acceleration pass over n massive bodies:
-    n (n - 1) ordered pairs at 28 FLOP each (the per-pair kick figure of the
-    test-particle count above); one pass per step, both half-kicks use it."""
-    return 28.0 * n * (n - 1)
+    n (n - 1) ordered pairs at 28 FLOP each (include/mint_flops.h); one pass
+    per step, both half-kicks use it."""
+    return mint.flops_per_kick_pass(n)
 
 
 def bench_embryos(args):

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/mint.h"
+#include "../../include/mint_flops.h"
 #include "mint_api_internal.h"
 
 using namespace mint;
@@ -994,5 +995,11 @@ int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const doub
   }
   return MINT_OK;
 }
+
+double mint_flops_per_tp_step(int32_t nm, double nbar) {
+  return 2.0 * nm * 28.0 + 12.0 + 12.0 + 9.0 * nm + 60.0 + 40.0 * nbar;
+}
+
+double mint_flops_per_kick_pass(int64_t n) { return 28.0 * (double)n * (double)(n - 1); }
 
 }  // extern "C"

@@ -104,6 +104,8 @@ _SIGS = {
     "mint_upload": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "mint_prepare": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mint_flops_per_tp_step": (C.c_double, [C.c_int32, C.c_double]),
+    "mint_flops_per_kick_pass": (C.c_double, [C.c_int64]),
     "mint_step_noalloc": (C.c_int, [C.c_void_p, C.c_int64]),
     "mint_download": (C.c_int, [C.c_void_p, C.POINTER(_State)]),
     "mint_energy": (C.c_int, [C.c_void_p, _DP]),
@@ -427,6 +429,16 @@ class Mint:
 
 
 # host helpers (no GPU needed) -------------------------------------------------
+def flops_per_tp_step(nbar: float, nm: int = 5) -> float:
+    """Algorithmic FLOP per test-particle step (include/mint_flops.h)."""
+    return float(lib().mint_flops_per_tp_step(int(nm), float(nbar)))
+
+
+def flops_per_kick_pass(n: int) -> float:
+    """Algorithmic FLOP of one massive kick-acceleration pass over n bodies."""
+    return float(lib().mint_flops_per_kick_pass(int(n)))
+
+
 def selftest_arith(op, a, b=None, device=0):
     """Device square root of squared distances (op 0), normal-range
     quotients a / b (op 1) or the kick's square root of positive finite

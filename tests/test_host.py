@@ -15,12 +15,12 @@ import mint_workloads as W
 import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "mint.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("mint.h", "mint_flops.h")]
 CLI = os.path.join(ROOT, "p9-integrator_amd", "mint")
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(mint_[a-z_0-9]+)\s*\(", text)))
 
@@ -36,6 +36,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_mirror_binds_every_declared_symbol():
     assert set(declared_symbols()) == set(mint._SIGS)
+
+
+def test_flop_counts_of_the_roofline():
+    # the per-unit figures the bench prices the kernels with: 529 FLOP per
+    # TP-step at 5 bodies and 3 Newton iterations, 28 per ordered massive pair
+    assert mint.flops_per_tp_step(3.0, 5) == 529.0
+    assert mint.flops_per_tp_step(1.0, 5) == 449.0
+    assert mint.flops_per_tp_step(0.0, 2) == 112 + 24 + 18 + 60
+    assert mint.flops_per_kick_pass(1000) == 28.0 * 1000 * 999
 
 
 def test_no_device_is_a_clean_error():
