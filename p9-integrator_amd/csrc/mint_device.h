@@ -415,6 +415,22 @@ __device__ __forceinline__ double close_weight(double r2, double eps2, double m,
   return div_normal(num, den);
 }
 
+// close_weight formed without branches: both forms are evaluated and
+// selected (the same values, NaN inputs included), so that it interleaves
+// with independent work instead of splitting the wavefront's instruction
+// stream at divergent branches.
+__device__ __forceinline__ double close_weight_sel(double r2, double eps2, double m, double rclo,
+                                                   double rcwinv) {
+  const double r = sqrt_dist2(r2);
+  const double yv = (r - rclo) * rcwinv;
+  const double D = (r2 + eps2) * r;
+  const double om = 1.0 - yv;
+  const double nb = m * (om * om), db = D * fma(2.0, yv * yv, fma(-2.0, yv, 1.0));
+  const bool band = yv > 0.0;
+  const double w = div_normal(band ? nb : m, band ? db : D);
+  return yv >= 1.0 ? 0.0 : w;
+}
+
 // TP kick acceleration from the NM massive bodies at positions X (body-major
 // xyz).  Replaces body_interaction_kick for a massless body (src/sim.cuh:327-382).
 // far: every pair is beyond its far radius (see P_FAR2).
@@ -523,8 +539,9 @@ __device__ __forceinline__ void perturber_pos(int j, double tau, int ci, const d
   }
 }
 
-// (1-K)-weighted pull of perturber j at (px, py, pz) on the TP at (x, y, z).
-template <int NM>
+// (1-K)-weighted pull of perturber j at (px, py, pz) on the TP at (x, y, z)
+// (SEL: the weight formed without branches, same values).
+template <int NM, bool SEL = false>
 __device__ __forceinline__ void pull_at(int j, double px, double py, double pz, double x, double y,
                                         double z, const double *__restrict__ prm, double eps2,
                                         double &w, double &dx, double &dy, double &dz) {
@@ -532,10 +549,13 @@ __device__ __forceinline__ void pull_at(int j, double px, double py, double pz, 
   dy = py - y;
   dz = pz - z;
   const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-  w = close_weight(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j]);
+  if (SEL)
+    w = close_weight_sel(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j]);
+  else
+    w = close_weight(d2, eps2, prm[P_M(NM) + j], prm[P_RCLO(NM) + j], prm[P_RCWINV(NM) + j]);
 }
 
-template <int NM>
+template <int NM, bool SEL = false>
 __device__ __forceinline__ void perturber_pull(int j, double tau, double x, double y, double z,
                                                const double *__restrict__ Y,
                                                const double *__restrict__ W,
@@ -545,7 +565,7 @@ __device__ __forceinline__ void perturber_pull(int j, double tau, double x, doub
                                                double &dz) {
   double px, py, pz;
   perturber_pos<NM>(j, tau, ci, Y, W, P4, mu, px, py, pz);
-  pull_at<NM>(j, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
+  pull_at<NM, SEL>(j, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
 }
 
 // Grouped right-hand side (G lanes per TP, see rhs_tp) with this lane's
@@ -563,7 +583,7 @@ __device__ __forceinline__ void rhs_tp_at(const double *yv, double px, double py
   const double w0 = div_normal(mu, r2 * r);
   double ax = -(w0 * x), ay = -(w0 * y), az = -(w0 * z);
   double w, dx, dy, dz;
-  pull_at<NM>(jl < NM ? jl : NM - 1, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
+  pull_at<NM, true>(jl < NM ? jl : NM - 1, px, py, pz, x, y, z, prm, eps2, w, dx, dy, dz);
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     const double wj = __shfl(w, base + j, 64);
@@ -602,7 +622,7 @@ __device__ __forceinline__ void rhs_tp(double tau, const double *yv, const doubl
     }
   } else {
     double w, dx, dy, dz;
-    perturber_pull<NM>(jl < NM ? jl : NM - 1, tau, x, y, z, Y, W, P4, ci, prm, mu, eps2, w, dx, dy, dz);
+    perturber_pull<NM, true>(jl < NM ? jl : NM - 1, tau, x, y, z, Y, W, P4, ci, prm, mu, eps2, w, dx, dy, dz);
 #pragma unroll
     for (int j = 0; j < NM; ++j) {
       const double wj = __shfl(w, base + j, 64);

@@ -955,6 +955,14 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   // BS extrapolation tableau of each lane, lane-interleaved in LDS (one-wave blocks)
   __shared__ double Rsh[KMAX * 6 * 64];
   double *Rt = Rsh + (threadIdx.x & 63u);
+  // G > 1: each group stages a step's ephemeris record and perturber cache
+  // in LDS, two steps deep (the next step's is loaded while this one runs)
+  constexpr int SST = G > 1 ? REC + pc_size(NM) : 1;
+  constexpr int NPF = G > 1 ? (SST + G - 1) / G : 1;
+  __shared__ double Sst[G > 1 ? 2 * (64 / G) * SST : 1];
+  // G > 1: and the entry's system parameter block, once per entry
+  constexpr int SPR = G > 1 ? prm_size(NM) : 1;
+  __shared__ double Spr[G > 1 ? (64 / G) * SPR : 1];
   // entries [qbegin, qcount) (qbegin null: from 0)
   const unsigned int e0 = qbegin ? *qbegin : 0u;
   const unsigned int e1 = *qcount;
@@ -978,6 +986,16 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
     const QEntry en = q[e0 + e];
     const int s = en.sys;
     const double *prm = prm_all + (size_t)s * prm_size(NM);
+    if (G > 1) {
+      // the parameter block read from LDS by every right-hand side, kick and
+      // check of the entry (the group's lanes write and read it in order)
+      double *d = Spr + (size_t)g * SPR;
+      for (int i = (int)el; i < SPR; i += G) d[i] = prm[i];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      prm = d;
+    }
     const double *rsys = rec + (size_t)s * C * REC;
     double x, y, z, vx, vy, vz;
     if (en.phase == 0) {  // queued from the window start: the state is in the TP arrays
@@ -1002,18 +1020,48 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       z = fma(sc.hh, r[R_Q1(NM) + 2], z);
       flag = encounter_tp<NM>(x, y, z, r + R_Y(NM), prm);
     }
+    // step data of step kk into registers: this lane's share (elements el,
+    // el + G, ...) of the record then the perturber cache
+    double pf[NPF];
+    double *Sg = Sst + (size_t)g * 2 * SST;
+    auto fetch = [&](int kk) {
+      const double *ra = rsys + (size_t)kk * REC;
+      const double *pa = pc + ((size_t)s * C + kk) * pc_size(NM);
+#pragma unroll
+      for (int u = 0; u < NPF; ++u) {
+        const int i = (int)el + u * G;
+        pf[u] = i < REC ? ra[i] : (i < SST ? pa[i - REC] : 0.0);
+      }
+    };
+    auto stage = [&](int kk) {
+      double *d = Sg + (kk & 1) * SST;
+#pragma unroll
+      for (int u = 0; u < NPF; ++u) {
+        const int i = (int)el + u * G;
+        if (i < SST) d[i] = pf[u];
+      }
+      // the group's lanes read what the others wrote: LDS operations of one
+      // wavefront complete in order, and nothing is moved across this point
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    if (G > 1) {
+      fetch(k);
+      stage(k);
+      if (k + 1 < C) fetch(k + 1);
+    }
     for (;;) {
-      const double *r = rsys + (size_t)k * REC;
+      const double *r = G > 1 ? Sg + (k & 1) * SST : rsys + (size_t)k * REC;
+      const double *pk = G > 1 ? r + REC : pc + ((size_t)s * C + k) * pc_size(NM);
       if (flag) {
         double yv[6] = {x, y, z, vx, vy, vz};
         int nr = 0;
         if (G == 1)
-          st |= bs_drift_tp<NM, 1>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
-                                   pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2, sc.tol,
+          st |= bs_drift_tp<NM, 1>(yv, sc.h, r + R_Y(NM), r + R_W(NM), pk, prm, sc.mu, sc.eps2, sc.tol,
                                    nr, Rt, 64);
         else
-          st |= bs_drift_tp_rows<NM, GP, R>(yv, sc.h, r + R_Y(NM), r + R_W(NM),
-                                            pc + ((size_t)s * C + k) * pc_size(NM), prm, sc.mu, sc.eps2,
+          st |= bs_drift_tp_rows<NM, GP, R>(yv, sc.h, r + R_Y(NM), r + R_W(NM), pk, prm, sc.mu, sc.eps2,
                                             sc.tol, nr, Rt, 64, (int)jl, (int)row, (int)ebase);
         x = yv[0]; y = yv[1]; z = yv[2]; vx = yv[3]; vy = yv[4]; vz = yv[5];
         n_bs += lead;
@@ -1031,7 +1079,11 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
       vz = fma(sc.hh, az, vz);
       n_step += lead;
       if (++k >= C) break;
-      const double *rn = rsys + (size_t)k * REC;
+      if (G > 1) {
+        stage(k);
+        if (k + 1 < C) fetch(k + 1);
+      }
+      const double *rn = G > 1 ? Sg + (k & 1) * SST : rsys + (size_t)k * REC;
       vx = fma(sc.hh, ax, vx);
       vy = fma(sc.hh, ay, vy);
       vz = fma(sc.hh, az, vz);

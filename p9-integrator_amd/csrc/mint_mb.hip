@@ -733,17 +733,6 @@ __global__ void __launch_bounds__(MB_BLOCK)
 // slots past the group are selected away, so that the partners' chains and
 // the Sun term interleave.  GM = 0: any group size, one member after the
 // other.  Same values either way.
-__device__ __forceinline__ double close_weight_sel(double r2, double eps2, double m, double rclo, double rcwinv) {
-  const double r = sqrt_dist2(r2);
-  const double yv = (r - rclo) * rcwinv;
-  const double D = (r2 + eps2) * r;
-  const double om = 1.0 - yv;
-  const double nb = m * (om * om), db = D * fma(2.0, yv * yv, fma(-2.0, yv, 1.0));
-  const bool band = yv > 0.0;
-  const double w = div_normal(band ? nb : m, band ? db : D);
-  return yv >= 1.0 ? 0.0 : w;
-}
-
 template <int GM>
 __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, double mu, double eps2,
                                   double tol, double mj, double loj, double wvj, unsigned int &nrhs) {
