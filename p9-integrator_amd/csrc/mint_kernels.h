@@ -616,21 +616,21 @@ __device__ __forceinline__ float scorer_rc(double rc2) { return __builtin_amdgcn
 // closest approach over the window at least 8 cut r_c).  Sufficient, without
 // a square root: r^2 >= 2 (A^2 + (|u| T)^2) implies r >= A + |u| T, and
 // |d + u t| >= |d| - |u| T over the window; A = 8 cut r_c and every term carry
-// a margin of 1e-3 over the FP32 roundings.  The differences are taken in
-// FP32 from FP32 positions and velocities (absolute error about 2^-24 of the
-// positions' size, inside the margin wherever r reaches 8 cut r_c of a
-// planet).  A pair it does not rule out is checked on its formed features;
-// the scorer only decides which TPs start the window in the queue, so the
-// test changes no result either way.
-__device__ __forceinline__ bool scorer_far(float x, float y, float z, float vx, float vy, float vz, float X,
-                                           float Y, float Z, float VX, float VY, float VZ, float rcf, float Tf,
-                                           float cut) {
-  const float dx = X - x, dy = Y - y, dz = Z - z;
-  const float ux = VX - vx, uy = VY - vy, uz = VZ - vz;
-  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
-  const float A = (8.0f * 1.001f) * cut * rcf, uT2 = (1.001f * 1.001f) * (Tf * Tf);
-  return r2 * 0.999f >= 2.0f * fmaf(u2, uT2, A * A);
+// a margin of 1e-3, far more than the FP32 roundings of the features (about
+// 1e-6 of the distances involved) can take away.  Formed in FP64 from the
+// body's values as they are (uniform operands where a wavefront's TPs share a
+// system), so no conversion is spent on a pair that turns out far.  A pair it
+// does not rule out is checked on its formed features; the scorer only
+// decides which TPs start the window in the queue, so the test changes no
+// result either way.  A2 = (8.008 cut)^2 r_c^2, uT2 = (1.001 T)^2.
+__device__ __forceinline__ bool scorer_far(double x, double y, double z, double vx, double vy, double vz,
+                                           double X, double Y, double Z, double VX, double VY, double VZ,
+                                           double A2, double uT2) {
+  const double dx = X - x, dy = Y - y, dz = Z - z;
+  const double ux = VX - vx, uy = VY - vy, uz = VZ - vz;
+  const double r2 = dot3(dx, dy, dz, dx, dy, dz);
+  const double u2 = dot3(ux, uy, uz, ux, uy, uz);
+  return r2 * 0.999 >= 2.0 * fma(u2, uT2, A2);
 }
 
 
@@ -806,6 +806,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 // changes.  With the trained weights cut is 1.5 (12 r_c); on the 4096-system
 // sweep about a quarter of a pair per TP is then scored.
 constexpr int PB_BLOCK = 256;
+// Pairs scored per pass of the block: the list is one pass long except where
+// a block has more pairs to score than it has lanes.
+constexpr int PB_CAP = PB_BLOCK;
 
 template <int NM>
 __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
@@ -814,23 +817,31 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     const float *__restrict__ scw, float thresh, float cut, unsigned char *__restrict__ carry,
     unsigned char *__restrict__ pre, QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
     unsigned int *__restrict__ need, int *__restrict__ slist, unsigned long long *__restrict__ stats) {
-  __shared__ float sh_f[PB_BLOCK * NM][SC_IN];  // packed pair features
-  __shared__ unsigned short sh_item[PB_BLOCK * NM];  // owner lane * 8 + body
-  __shared__ float sh_logit[PB_BLOCK][NM];
+  // 6.8 KB of LDS per block (the packed list of one pass, each entry's owner
+  // lane, a pick flag per lane), so that LDS does not cap the blocks per CU
+  __shared__ float sh_f[PB_CAP][SC_IN];
+  __shared__ unsigned short sh_item[PB_CAP];
+  __shared__ unsigned char sh_go[PB_BLOCK];
   __shared__ unsigned int sh_n;
   const int t = (int)threadIdx.x;
   const long long i = (long long)blockIdx.x * PB_BLOCK + t;
   unsigned int n_pre = 0;
-  bool go = false;
-  int s = -1;
-  if (i < n) {
-    go = carry[i] != 0;
-    if (go) carry[i] = 0;
+  // Every per-lane load is issued here, at once (one round trip instead of a
+  // chain of carry flag, system index and state).  The state of a carried TP
+  // is loaded too but not used: its previous window may still be writing it.
+  const long long ic = i < n ? i : n - 1;
+  const bool carried = carry[ic] != 0;
+  int s = tp_sys[ic];
+  double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  if (scw != nullptr) {
+    x = tp.x[ic]; y = tp.y[ic]; z = tp.z[ic];
+    vx = tp.vx[ic]; vy = tp.vy[ic]; vz = tp.vz[ic];
   }
+  bool go = i < n && carried;
+  if (go) carry[i] = 0;
   if (scw != nullptr) {  // uniform: every lane takes part in the block's phases
     if (t == 0) sh_n = 0u;
-#pragma unroll
-    for (int j = 0; j < NM; ++j) sh_logit[t][j] = -INFINITY;
+    sh_go[t] = 0;
     float f[NM][SC_IN];
     unsigned int want = 0u;
     const bool mine = i < n && !go;
@@ -838,27 +849,25 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     auto pairs = [&](int sys) {
       const double *prm = prm_all + (size_t)sys * prm_size(NM);
       const double *bx = wx + (size_t)sys * 3 * NM, *bv = wv + (size_t)sys * 3 * NM;
-      const double x = tp.x[i], y = tp.y[i], z = tp.z[i], vx = tp.vx[i], vy = tp.vy[i], vz = tp.vz[i];
-      const float xf = (float)x, yf = (float)y, zf = (float)z, vxf = (float)vx, vyf = (float)vy, vzf = (float)vz;
       const double T = sc.h * (double)C;
+      const double uT2 = (1.001 * 1.001) * (T * T);
+      const double a2 = (8.008 * 8.008) * ((double)cut * (double)cut);
 #pragma unroll
       for (int j = 0; j < NM; ++j) {
-        const float rcf = scorer_rc(prm[P_RC2(NM) + j]);
-        if (cut <= 8.0f && scorer_far(xf, yf, zf, vxf, vyf, vzf, (float)bx[3 * j], (float)bx[3 * j + 1],
-                                      (float)bx[3 * j + 2], (float)bv[3 * j], (float)bv[3 * j + 1],
-                                      (float)bv[3 * j + 2], rcf, (float)T, cut))
+        const double rc2 = prm[P_RC2(NM) + j];
+        if (cut <= 8.0f && scorer_far(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
+                                      bv[3 * j + 1], bv[3 * j + 2], a2 * rc2, uT2))
           continue;
         scorer_features(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
-                        bv[3 * j + 1], bv[3 * j + 2], rcf, T, f[j]);
+                        bv[3 * j + 1], bv[3 * j + 2], scorer_rc(rc2), T, f[j]);
         if (!(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
       }
     };
     // TPs are stored by system, so a wavefront's lanes nearly always share
     // one: then the system's bodies come through scalar loads, once
-    if (mine) s = tp_sys[i];
     const unsigned long long mm = __ballot(mine);
     const int lead = mm ? __ffsll((long long)mm) - 1 : 0;
-    const int s0 = __builtin_amdgcn_readfirstlane(__shfl(s, lead, 64));
+    const int s0 = __builtin_amdgcn_readlane(s, lead);
     const bool one = mm != 0ull && __ballot(mine && s != s0) == 0ull;
     if (one) {
       if (mine) pairs(s0);
@@ -874,51 +883,56 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
       if ((t & 63) >= off) incl += o;
     }
     const unsigned int wtot = __shfl(incl, 63, 64);
-    __syncthreads();  // sh_n and sh_logit initialised
+    __syncthreads();  // sh_n and sh_go initialised
     unsigned int wbase = 0u;
     if ((t & 63) == 63) wbase = atomicAdd(&sh_n, wtot);
     wbase = __shfl(wbase, 63, 64);
-    unsigned int slot = wbase + incl - cnt;
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-      if (!((want >> j) & 1u)) continue;
-#pragma unroll
-      for (int k = 0; k < SC_IN; ++k) sh_f[slot][k] = f[j][k];
-      sh_item[slot] = (unsigned short)(t * 8 + j);
-      ++slot;
-    }
+    const unsigned int slot0 = wbase + incl - cnt;
     __syncthreads();
     const unsigned int ni = sh_n;
-    for (unsigned int it = (unsigned int)t; it < ni; it += PB_BLOCK) {
-      float g[SC_IN];
+    // passes over the list, PB_CAP entries each: a lane writes the entries
+    // of its pairs that fall in the pass, lane e scores entry e, and a pair
+    // above the threshold flags its owner (a pair above it is what the
+    // largest logit above it means; a NaN logit flags nothing either way)
+    for (unsigned int p0 = 0; p0 < ni; p0 += PB_CAP) {
+      unsigned int sl = slot0;
 #pragma unroll
-      for (int k = 0; k < SC_IN; ++k) g[k] = sh_f[it][k];
-      const unsigned int own = sh_item[it];
-      // scorer_fwd's sums in its order, one hidden unit's 8 weights loaded
-      // at a time (all 129 held at once would spill from scalar registers)
-      float out = scw[SC_B2];
-#pragma unroll 2
-      for (int u = 0; u < SC_H; ++u) {
-        float a = scw[SC_B1 + u];
+      for (int j = 0; j < NM; ++j) {
+        if (!((want >> j) & 1u)) continue;
+        if (sl >= p0 && sl < p0 + PB_CAP) {
 #pragma unroll
-        for (int k = 0; k < SC_IN; ++k) a = fmaf(scw[SC_W1 + u * SC_IN + k], g[k], a);
-        out = fmaf(scw[SC_W2 + u], nn_tanh(a), out);
+          for (int k = 0; k < SC_IN; ++k) sh_f[sl - p0][k] = f[j][k];
+          sh_item[sl - p0] = (unsigned short)t;
+        }
+        ++sl;
       }
-      sh_logit[own >> 3][own & 7u] = out;
-    }
-    __syncthreads();
-    if (i < n && !go) {
-      float best = -INFINITY;
+      __syncthreads();
+      if ((unsigned int)t < ni - p0) {
+        float g[SC_IN];
 #pragma unroll
-      for (int j = 0; j < NM; ++j) best = fmaxf(best, sh_logit[t][j]);
-      go = best > thresh;
+        for (int k = 0; k < SC_IN; ++k) g[k] = sh_f[t][k];
+        // scorer_fwd's sums in its order, one hidden unit's 8 weights loaded
+        // at a time (all 129 held at once would spill from scalar registers)
+        float out = scw[SC_B2];
+#pragma unroll 2
+        for (int u = 0; u < SC_H; ++u) {
+          float a = scw[SC_B1 + u];
+#pragma unroll
+          for (int k = 0; k < SC_IN; ++k) a = fmaf(scw[SC_W1 + u * SC_IN + k], g[k], a);
+          out = fmaf(scw[SC_W2 + u], nn_tanh(a), out);
+        }
+        if (out > thresh) sh_go[sh_item[t]] = 1;
+      }
+      __syncthreads();
+    }
+    if (mine) {
+      go = sh_go[t] != 0;
       n_pre = go ? 1u : 0u;
     }
   }
   if (i < n) {
     pre[i] = go ? 1 : 0;
     if (go) {
-      if (s < 0) s = tp_sys[i];
       const unsigned int qs = atomicAdd(qcount, 1u);
       QEntry *e = q + qs;
       e->tp = (int)i;

@@ -8,6 +8,9 @@
         dispatch of the kernels matching the substring
     python prof_summary.py csvcounter <counter_collection.csv> [kernel-substring]
         every dispatch's counter values (CSV), for the kernels matching
+    python prof_summary.py csvtimeline <kernel_trace.csv> <kernel-substring> [index] [pad-ms]
+        the dispatches around one dispatch of the matching kernels (default
+        the last), with start and end relative to it and their queues
 """
 import csv
 import sqlite3
@@ -66,6 +69,25 @@ def csvtrace(path, filt=""):
                 print(f"  `{k}`: " + ", ".join(f"{x:.4f}" for x in v))
 
 
+def csvtimeline(path, anchor, which="-1", pad_ms="0.6"):
+    """Every dispatch that starts within pad_ms of the which-th dispatch of the
+    kernels matching anchor (default: the last one), in start order, with
+    start and end relative to the anchor's start (ms) and its queue."""
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    hits = [r for r in rows if anchor in r["Kernel_Name"]]
+    a = hits[int(which)]
+    t0, t1 = int(a["Start_Timestamp"]), int(a["End_Timestamp"])
+    pad = float(pad_ms) * 1e6
+    print(f"| kernel | queue | start ms | end ms | ms |")
+    print("|---|---|---|---|---|")
+    for r in rows:
+        s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if t0 - pad <= s <= t1 + pad:
+            print(f"| `{short(r['Kernel_Name'])}` | {r.get('Queue_Id', r.get('Stream_Id', '?'))} | "
+                  f"{(s - t0) / 1e6:.4f} | {(e - t0) / 1e6:.4f} | {(e - s) / 1e6:.4f} |")
+
+
 def csvcounter(path, filt=""):
     rows = list(csv.DictReader(open(path)))
     per = defaultdict(dict)
@@ -80,4 +102,5 @@ if __name__ == "__main__":
     cmd, path = sys.argv[1], sys.argv[2]
     filt = sys.argv[3] if len(sys.argv) > 3 else ""
     {"trace": lambda: trace(path), "counter": lambda: counter(path, filt), "csvtrace": lambda: csvtrace(path, filt),
-     "csvcounter": lambda: csvcounter(path, filt)}[cmd]()
+     "csvcounter": lambda: csvcounter(path, filt),
+     "csvtimeline": lambda: csvtimeline(path, filt, *sys.argv[4:])}[cmd]()
