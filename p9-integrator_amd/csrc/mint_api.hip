@@ -514,7 +514,7 @@ int upload_impl(Ctx *c, const mint_state *st) {
   if ((rc = c->dalloc((void **)&c->d_need, (size_t)2 * S * sizeof(unsigned int), "cache flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_slist, (size_t)4 * S * sizeof(int), "cache system lists"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_q, (size_t)std::max<int64_t>(N, 1) * sizeof(QEntry), "BS queue"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_qctl, 8 * sizeof(unsigned int), "queue counters"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_qctl, 16 * sizeof(unsigned int), "queue counters"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_kep, KM_NP * sizeof(float), "kepler mlp"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_nnw_sc, SC_NP * sizeof(float), "scorer"))) return rc;
   }
@@ -539,7 +539,9 @@ int upload_impl(Ctx *c, const mint_state *st) {
     c->timing = mint_timing{};
     return MINT_OK;
   }
-  if ((rc = c->hip(hipMemset(c->d_qctl, 0, 8 * sizeof(unsigned int)), "queue clear"))) return rc;
+  // the queues start clear (a batched call clears parity 0's again at its end)
+  if ((rc = c->hip(hipMemset(c->d_qctl, 0, 16 * sizeof(unsigned int)), "queue clear"))) return rc;
+  if ((rc = c->hip(hipMemset(c->d_need, 0, (size_t)2 * S * sizeof(unsigned int)), "cache flag clear"))) return rc;
   if (N > 0 && (rc = c->hip(hipMemset(c->d_carry, 0, (size_t)N), "carry clear"))) return rc;
   // networks: keep any weights set earlier on this context
   if (!c->w_kep.empty())

@@ -86,8 +86,9 @@ struct Ctx {
   int ntiles = 0;
   long long tiles_key = -1;
   QEntry *d_q = nullptr, *d_q2 = nullptr;  // BS queues of even / odd windows
-  // [2][4] per window parity: [0] entries so far, [1] systems flagged 1,
-  // [2] entries of the queue head, [3] systems flagged 2
+  // [2][8] per window parity: [0] entries of the queue head, [1] systems
+  // flagged 1, [3] systems flagged 2, [4] mid-window deferrals (queued after
+  // the head); [2], [5..7] unused
   unsigned int *d_qctl = nullptr;
   unsigned int *d_status = nullptr;
   unsigned long long *d_stats = nullptr;

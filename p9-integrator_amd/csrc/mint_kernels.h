@@ -310,15 +310,16 @@ __device__ __noinline__ uint32_t massive_groups(double *S6, const uint32_t *adj,
 
 // The massive subsystem of systems [s_begin, s_end) for C steps: the
 // per-step ephemeris records every TP phase reads (replaces the cluster-wide
-// reads of block 0's shared memory, src/sim.cuh:15-26,682-741), then the
-// state at the end of the C steps, into (mx, mv) and, when nx is not null,
-// also into (nx, nv): the next launch window's start, which the queue head of
-// that window reads.  One-wave blocks of 64 / EPH_L systems.
+// reads of block 0's shared memory, src/sim.cuh:15-26,682-741), from the
+// start state (ix, iv); the state at the end of the C steps goes into (mx, mv)
+// when mx is not null (ix = mx: in place) and into (nx, nv) when nx is not
+// null (the next launch window's start, which the queue head of that window
+// reads).  One-wave blocks of 64 / EPH_L systems.
 template <int NM>
 __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, int C,
                                                        const double *__restrict__ prm_all,
-                                                       double *__restrict__ mx,
-                                                       double *__restrict__ mv,
+                                                       const double *ix, const double *iv,
+                                                       double *mx, double *mv,
                                                        double *__restrict__ xs0,
                                                        double *__restrict__ rec,
                                                        double *__restrict__ nx,
@@ -345,8 +346,8 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
   double x[3], v[3], a[3], Q[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    x[c] = mx[((size_t)s * NM + jb) * 3 + c];
-    v[c] = mv[((size_t)s * NM + jb) * 3 + c];
+    x[c] = ix[((size_t)s * NM + jb) * 3 + c];
+    v[c] = iv[((size_t)s * NM + jb) * 3 + c];
     if (own) xs0[((size_t)s * NM + jb) * 3 + c] = x[c];
   }
   accel_lane<NM>(x, base, jb, lp, sc.eps2, a);
@@ -449,8 +450,10 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
   bool fin = true;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    mx[((size_t)s * NM + jb) * 3 + c] = x[c];
-    mv[((size_t)s * NM + jb) * 3 + c] = v[c];
+    if (mx) {
+      mx[((size_t)s * NM + jb) * 3 + c] = x[c];
+      mv[((size_t)s * NM + jb) * 3 + c] = v[c];
+    }
     if (nx) {
       nx[((size_t)s * NM + jb) * 3 + c] = x[c];
       nv[((size_t)s * NM + jb) * 3 + c] = v[c];
@@ -734,7 +737,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
         y = fma(hh, r[R_Q1(NM) + 1], y);
         z = fma(hh, r[R_Q1(NM) + 2], z);
         if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
-          const unsigned int slot = atomicAdd(qcount, 1u);
+          // after the queue head (qcount[0], final before this kernel), by
+          // the count of mid-window deferrals (qcount[4])
+          const unsigned int slot = qcount[0] + atomicAdd(qcount + 4, 1u);
           q_put(q + slot, (int)idx[p], s, k, 1, x, y, z, vx, vy, vz);
           flag_system(need, 2u, s, slist, qcount + 3);
           if (carry != nullptr) carry[idx[p]] = 1;  // queued for the next window too
@@ -977,9 +982,9 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   // G > 1: and the entry's system parameter block, once per entry
   constexpr int SPR = G > 1 ? prm_size(NM) : 1;
   __shared__ double Spr[G > 1 ? (64 / G) * SPR : 1];
-  // entries [qbegin, qcount) (qbegin null: from 0)
+  // entries [0, *qcount) (qbegin null), or *qcount entries from *qbegin
   const unsigned int e0 = qbegin ? *qbegin : 0u;
-  const unsigned int e1 = *qcount;
+  const unsigned int e1 = qbegin ? e0 + *qcount : *qcount;
   const unsigned int n = e1 > e0 ? e1 - e0 : 0u;
   unsigned int n_kep = 0, n_newton = 0, n_bs = 0, n_rhs = 0, n_step = 0;
   const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
@@ -1124,17 +1129,18 @@ __global__ void __launch_bounds__(64) tp_encounter_kernel(
   }
 }
 
-static __global__ void copy_u32_kernel(unsigned int *dst, const unsigned int *src) { *dst = *src; }
 
-// Start of a launch window: clears the window's queue counters ctl[4] and
-// system flags need[S]; for a call's first window (n > 0) also copies the
-// window-start massive state (the ephemeris writes it for every later window).
+// Clears a queue's counters ctl[8] and system flags need[S] (at the start of
+// every launch window but a call's first) and, with n > 0, copies n doubles
+// of massive state from (sx, sv) to (dx, dv) (at a call's end: the last
+// window's end state into the resident buffers, and parity 0's queue cleared
+// for the next call's first window).
 static __global__ void window_start_kernel(unsigned int *__restrict__ ctl, unsigned int *__restrict__ need, int S,
                                            double *__restrict__ dx, double *__restrict__ dv,
                                            const double *__restrict__ sx, const double *__restrict__ sv,
                                            long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 4) ctl[i] = 0u;
+  if (i < 8) ctl[i] = 0u;
   if (i < S) need[i] = 0u;
   if (i < n) {
     dx[i] = sx[i];

@@ -9,7 +9,8 @@ namespace mint {
 
 struct Launchers {
   hipError_t (*ephemeris)(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
-                          double *mx, double *mv, double *xs0, double *rec, double *nx, double *nv,
+                          const double *ix, const double *iv, double *mx, double *mv, double *xs0,
+                          double *rec, double *nx, double *nv,
                           unsigned int *status, unsigned long long *stats, StepConst sc,
                           double M);
   hipError_t (*tp_step)(hipStream_t st, int guess, int block, long long i0, long long i1,

@@ -11,12 +11,13 @@ namespace {
 constexpr int NMV = MINT_NM;
 
 hipError_t ephemeris_l(hipStream_t st, int s_begin, int s_end, int C, const double *prm,
-                       double *mx, double *mv, double *xs0, double *rec, double *nx, double *nv,
+                       const double *ix, const double *iv, double *mx, double *mv, double *xs0,
+                       double *rec, double *nx, double *nv,
                        unsigned int *status, unsigned long long *stats, StepConst sc, double M) {
   const int n = s_end - s_begin;
   if (n <= 0) return hipSuccess;
   constexpr int spw = 64 / EPH_L<NMV>;  // systems per one-wave block
-  ephemeris_kernel<NMV><<<(n + spw - 1) / spw, 64, 0, st>>>(s_begin, s_end, C, prm, mx, mv, xs0, rec,
+  ephemeris_kernel<NMV><<<(n + spw - 1) / spw, 64, 0, st>>>(s_begin, s_end, C, prm, ix, iv, mx, mv, xs0, rec,
                                                             nx, nv, status, stats, sc, M);
   return hipGetLastError();
 }

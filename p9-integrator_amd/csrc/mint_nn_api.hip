@@ -518,7 +518,8 @@ int mint_trace_encounter(void *ctx, int64_t nsteps, int64_t cap, float *feat, fl
     if (!cmx.alloc(mb) || !cmv.alloc(mb)) return c->fail(MINT_ENOMEM, "trace copy");
     hipMemcpyAsync(cmx.p, c->d_mx, mb, hipMemcpyDeviceToDevice, c->stream);
     hipMemcpyAsync(cmv.p, c->d_mv, mb, hipMemcpyDeviceToDevice, c->stream);
-    rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, C, c->d_prm, (double *)cmx.p, (double *)cmv.p, c->d_xs0,
+    rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, C, c->d_prm, (const double *)cmx.p, (const double *)cmv.p,
+                                (double *)cmx.p, (double *)cmv.p, c->d_xs0,
                                 c->d_rec, nullptr, nullptr, c->d_status, nullptr, sc, c->cfg.central_mass),
                 "trace ephemeris");
     if (rc) return rc;

@@ -87,7 +87,7 @@ extern "C" int mint_step_noalloc(void *ctx, int64_t nsteps) {
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
   int rc = c->hip(hipEventRecord(c->ev0, c->stream), "event");
   for (int64_t k = 0; k < nsteps && !rc; ++k) {
-    rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, 1, c->d_prm, c->d_mx, c->d_mv, xs0, rec, nullptr, nullptr,
+    rc = c->hip(c->L->ephemeris(c->stream, 0, c->S, 1, c->d_prm, c->d_mx, c->d_mv, c->d_mx, c->d_mv, xs0, rec, nullptr, nullptr,
                                 c->d_status, stats, sc, c->cfg.central_mass),
                 "ephemeris launch");
     if (!rc)

@@ -46,15 +46,16 @@ int enc_grid(const Ctx *c) {
 
 
 
-// Ephemeris of one launch window into buffer (xs0, rec) on stream st; the
-// window's end state also goes to the snapshot `next` (positions, then
-// velocities).
-int window_eph(Ctx *c, hipStream_t st, int C, double *xs0, double *rec, double *next, KTimer *kt) {
+// Ephemeris of one launch window into buffer (xs0, rec) on stream st, from the
+// massive state (in, in + nb) (positions, then velocities) to the snapshot
+// `next`, the window's end state.
+int window_eph(Ctx *c, hipStream_t st, int C, const double *in, const double *in_v, double *xs0, double *rec,
+               double *next, KTimer *kt) {
   const StepConst sc = c->step_const();
   if (kt) kt->start();
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
   const size_t nb = (size_t)c->S * c->NM * 3;
-  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, c->d_mx, c->d_mv, xs0, rec, next, next + nb,
+  int rc = c->hip(c->L->ephemeris(st, 0, c->S, C, c->d_prm, in, in_v, nullptr, nullptr, xs0, rec, next, next + nb,
                                   c->d_status, stats, sc, c->cfg.central_mass),
                   "ephemeris launch");
   if (rc) return rc;
@@ -76,18 +77,25 @@ int window_event(Ctx *c, std::vector<hipEvent_t> &v, size_t i, hipEvent_t *out) 
 
 // Enqueue nsteps of the batched form as launch windows of up to sh.C steps.
 //
-// Window w, on the context stream: reset of the queue of parity w%2, the
-// queue-head pass (TPs carried from window w-1, plus the scorer's picks),
-// then the main kernel, whose mid-window deferrals are queued for their tail
-// and carried into window w+1.  On the queue stream, forked after the head
-// pass: the tails of window w-1 (perturber cache, encounter kernel), then the
-// head of window w (cache, encounter kernel); joined back before window w+1.
-// A tail thus runs beside the next main kernel instead of after its own.  The
-// ephemeris stream computes window w into buffer w%3 while window w-1 runs;
-// that buffer was last read by window w-3 and by its tails, served in window
-// w-2.  The last window's tails run on the context stream at the end.
-// Without pipelining (or when timing each kernel) the same launches run in
-// the same order on the context stream.
+// Window w, on the context stream: reset of the queue of parity w%2 (for
+// w > 0), the queue-head pass (TPs carried from window w-1, plus the scorer's
+// picks), then the main kernel, whose mid-window deferrals are queued for
+// their tail and carried into window w+1.  On the queue stream, forked after
+// the head pass: the tails of window w-1 (perturber cache, encounter kernel),
+// then the head of window w (cache, encounter kernel); joined back before
+// window w+1.  A tail thus runs beside the next main kernel instead of after
+// its own.  The ephemeris stream computes window w into buffer w%3 while
+// window w-1 runs; that buffer was last read by window w-3 and by its tails,
+// served in window w-2.  The last window's tails run on the context stream at
+// the end.  Without pipelining (or when timing each kernel) the same launches
+// run in the same order on the context stream.
+//
+// Nothing writes the resident massive state (d_mx, d_mv) until the call's
+// last kernel: window 0's ephemeris and queue head read it as the call's start
+// state, each window's ephemeris writes its end state to the next window's
+// snapshot, and the last kernel copies the last snapshot back and clears the
+// queue of parity 0 for the next call (upload leaves it clear).  So the first
+// ephemeris and the first queue head depend on nothing and start at once.
 int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTimer *kt, bool capturing) {
   std::vector<int> win;
   for (int64_t done = 0; done < nsteps;) {
@@ -108,7 +116,10 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
   double *xs[3] = {c->d_xs0, c->d_xs0b, c->d_xs0c};
   double *rec[3] = {c->d_rec, c->d_recb, c->d_recc};
   QEntry *q[2] = {c->d_q, c->d_q2};
-  unsigned int *ctl[2] = {c->d_qctl, c->d_qctl + 4};
+  // per parity: [0] queue-head entries, [1] systems listed by the head, [3]
+  // systems listed by the tails, [4] mid-window deferrals (the tails, queued
+  // after the head)
+  unsigned int *ctl[2] = {c->d_qctl, c->d_qctl + 8};
   unsigned int *need[2] = {c->d_need, c->d_need + S};
   int *slist_head[2] = {c->d_slist, c->d_slist + 2 * S};
   int *slist_tail[2] = {c->d_slist + S, c->d_slist + 3 * S};
@@ -148,42 +159,38 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
                    "perturber cache launch");
     if (r) return r;
     const int grid = dense ? std::min(enc_grid(c), 256) : enc_grid(c);
-    r = c->hip(c->L->tp_encounter(st, guess, grid, 64, q[pu], tail ? ctl[pu] : ctl[pu] + 2,
-                                  tail ? ctl[pu] + 2 : nullptr, dense, !dense, tp, c->d_prm, xs[bu], rec[bu], c->d_pc,
+    r = c->hip(c->L->tp_encounter(st, guess, grid, 64, q[pu], tail ? ctl[pu] + 4 : ctl[pu],
+                                  tail ? ctl[pu] : nullptr, dense, !dense, tp, c->d_prm, xs[bu], rec[bu], c->d_pc,
                                   win[u], sc, nnw, c->d_status, stats),
                "encounter launch");
     if (r) return r;
     if (kt) kt->stop(c->timing.ms_encounter, c->timing.n_encounter);
     return MINT_OK;
   };
-  // the first window's start state for its queue head (later windows' come
-  // from the ephemeris of the window before), with its queue reset
   const long long nb = (long long)S * c->NM * 3;
-  window_start_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, sm>>>(ctl[0], need[0], S, c->d_snap[0],
-                                                                    c->d_snap[0] + nb, c->d_mx, c->d_mv, nb);
-  if ((rc = c->hip(hipGetLastError(), "snapshot launch"))) return rc;
   if ((rc = record(c->ev_side, 0, sm)) || (rc = wait(sp, c->ev_side, 0))) return rc;
   for (int w = 0; w < W; ++w) {
     const int p = w % 2, B = w % 3, C = win[w];
-    // ephemeris of window w (its end state is window w+1's snapshot, buffer
-    // (w+1) mod 3, last read by the queue head of window w-2)
-    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
-    if ((rc = window_eph(c, sp, C, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
-    if ((rc = record(c->ev_eph, w, sp))) return rc;
-    // queue head of window w, from the window-start snapshot: beside the
-    // window's ephemeris (window 0's queue was reset with its snapshot)
+    const double *wx = w == 0 ? c->d_mx : c->d_snap[B];  // window start state
+    const double *wv = w == 0 ? c->d_mv : c->d_snap[B] + nb;
+    // queue head of window w, from the window-start state, beside the
+    // window's ephemeris; enqueued first, as it is the longer of the two
+    // (window 0's queue was cleared by the previous call or the upload)
     if (w > 0) {
       window_start_kernel<<<(unsigned)((S + 255) / 256), 256, 0, sm>>>(ctl[p], need[p], S, nullptr, nullptr,
                                                                        nullptr, nullptr, 0);
       if ((rc = c->hip(hipGetLastError(), "queue reset launch"))) return rc;
     }
-    if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, c->d_snap[B], c->d_snap[B] + nb, C, sc, scw,
+    if ((rc = c->hip(c->L->tp_prebucket(sm, c->Ntp, c->d_tp_sys, tp, c->d_prm, wx, wv, C, sc, scw,
                                         c->score_thresh, c->scorer_skip_on() ? c->sc_cut : INFINITY, c->d_carry, c->d_pre, q[p], ctl[p], need[p],
                                         slist_head[p], stats),
                      "queue head launch")))
       return rc;
-    copy_u32_kernel<<<1, 1, 0, sm>>>(ctl[p] + 2, ctl[p]);
-    if ((rc = c->hip(hipGetLastError(), "queue head count launch"))) return rc;
+    // ephemeris of window w (its end state is window w+1's snapshot, buffer
+    // (w+1) mod 3, last read by the queue head of window w-2)
+    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
+    if ((rc = window_eph(c, sp, C, wx, wv, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
+    if ((rc = record(c->ev_eph, w, sp))) return rc;
     if ((rc = wait(sm, c->ev_eph, w))) return rc;
     // queue stream: tails of window w-1, then the head of window w
     if ((rc = record(c->ev_fork, w, sm)) || (rc = wait(sq, c->ev_fork, w))) return rc;
@@ -205,7 +212,12 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
   }
   if (ktime && c->ntiles > 0) c->kwin_pending = W;
   // the last window's tails
-  return serve(W - 1, true, sm);
+  if ((rc = serve(W - 1, true, sm))) return rc;
+  // the end state back into the resident buffers; parity 0's queue cleared
+  double *last = c->d_snap[W % 3];
+  window_start_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, sm>>>(ctl[0], need[0], S, c->d_mx, c->d_mv, last,
+                                                                    last + nb, nb);
+  return c->hip(hipGetLastError(), "call end launch");
 }
 
 // The cached graph of an nsteps call at shape sh, captured now if there is
@@ -333,7 +345,7 @@ int run_per_step(Ctx *c, int64_t nsteps, bool flat) {
     for (int s0 = 0; s0 < c->S; s0 += spl) {
       const int s1 = std::min(c->S, s0 + spl);
       kt.start();
-      rc = c->hip(c->L->ephemeris(c->stream, s0, s1, 1, c->d_prm, c->d_mx, c->d_mv, c->d_xs0, c->d_rec,
+      rc = c->hip(c->L->ephemeris(c->stream, s0, s1, 1, c->d_prm, c->d_mx, c->d_mv, c->d_mx, c->d_mv, c->d_xs0, c->d_rec,
                                   nullptr, nullptr, c->d_status, stats, sc, c->cfg.central_mass),
                   "ephemeris launch");
       if (rc) return rc;
