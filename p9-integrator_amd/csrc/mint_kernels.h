@@ -584,12 +584,19 @@ __global__ void __launch_bounds__(256) tp_step_kernel(long long i0, long long i1
 // (positions are hundreds of AU); the rest is FP32 with the hardware square
 // root, reciprocal square root and reciprocal (about one ulp each): the scorer
 // only ranks pairs, and the exact distance check decides every step.
-__device__ __forceinline__ void scorer_features(double x, double y, double z, double vx, double vy,
-                                                double vz, double X, double Y, double Z, double VX,
-                                                double VY, double VZ, float rcf, double T,
-                                                float *f) {
-  const float dx = (float)(X - x), dy = (float)(Y - y), dz = (float)(Z - z);
-  const float ux = (float)(VX - vx), uy = (float)(VY - vy), uz = (float)(VZ - vz);
+// scorer_tp_r: the TP's heliocentric distance in FP32, the same for all its
+// pairs; scorer_features_d takes the pair's FP64 differences (body - TP) as
+// formed by the caller.
+__device__ __forceinline__ float scorer_tp_r(double x, double y, double z) {
+  const float xf = (float)x, yf = (float)y, zf = (float)z;
+  return __builtin_amdgcn_sqrtf(fmaf(xf, xf, fmaf(yf, yf, zf * zf)));
+}
+
+__device__ __forceinline__ void scorer_features_d(double ddx, double ddy, double ddz, double dux, double duy,
+                                                  double duz, double X, double Y, double Z, float rt, float rcf,
+                                                  double T, float *f) {
+  const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+  const float ux = (float)dux, uy = (float)duy, uz = (float)duz;
   const float Tf = (float)T, irc = __builtin_amdgcn_rcpf(rcf);
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   const float u2 = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
@@ -600,15 +607,20 @@ __device__ __forceinline__ void scorer_features(double x, double y, double z, do
   const float mx = fmaf(ux, tmin, dx), my = fmaf(uy, tmin, dy), mz = fmaf(uz, tmin, dz);
   const float rmin = __builtin_amdgcn_sqrtf(fmaf(mx, mx, fmaf(my, my, mz * mz)));
   const float Xf = (float)X, Yf = (float)Y, Zf = (float)Z;
-  const float xf = (float)x, yf = (float)y, zf = (float)z;
   const float iR = __builtin_amdgcn_rsqf(fmaf(Xf, Xf, fmaf(Yf, Yf, Zf * Zf)));
-  const float rt = __builtin_amdgcn_sqrtf(fmaf(xf, xf, fmaf(yf, yf, zf * zf)));
   f[0] = fminf(r * irc, 64.0f) * 0.125f;
   f[1] = fmaxf(fminf(((du * __builtin_amdgcn_rcpf(r)) * irc) * Tf, 16.0f), -16.0f) * 0.25f;
   f[2] = fminf((__builtin_amdgcn_sqrtf(u2) * Tf) * irc, 16.0f) * 0.25f;
   f[3] = fminf(rmin * irc, 64.0f) * 0.125f;
   f[4] = fminf(rcf * iR, 1.0f) * 4.0f;
   f[5] = fminf(rt * iR, 8.0f) * 0.25f;
+}
+
+__device__ __forceinline__ void scorer_features(double x, double y, double z, double vx, double vy,
+                                                double vz, double X, double Y, double Z, double VX,
+                                                double VY, double VZ, float rcf, double T,
+                                                float *f) {
+  scorer_features_d(X - x, Y - y, Z - z, VX - vx, VY - vy, VZ - vz, X, Y, Z, scorer_tp_r(x, y, z), rcf, T, f);
 }
 
 // r_c of pair body j in FP32, from r_c^2 (the scorer's unit of length).
@@ -626,11 +638,8 @@ __device__ __forceinline__ float scorer_rc(double rc2) { return __builtin_amdgcn
 // does not rule out is checked on its formed features; the scorer only
 // decides which TPs start the window in the queue, so the test changes no
 // result either way.  A2 = (8.008 cut)^2 r_c^2, uT2 = (1.001 T)^2.
-__device__ __forceinline__ bool scorer_far(double x, double y, double z, double vx, double vy, double vz,
-                                           double X, double Y, double Z, double VX, double VY, double VZ,
+__device__ __forceinline__ bool scorer_far(double dx, double dy, double dz, double ux, double uy, double uz,
                                            double A2, double uT2) {
-  const double dx = X - x, dy = Y - y, dz = Z - z;
-  const double ux = VX - vx, uy = VY - vy, uz = VZ - vz;
   const double r2 = dot3(dx, dy, dz, dx, dy, dz);
   const double u2 = dot3(ux, uy, uz, ux, uy, uz);
   return r2 * 0.999 >= 2.0 * fma(u2, uT2, A2);
@@ -857,14 +866,16 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
       const double T = sc.h * (double)C;
       const double uT2 = (1.001 * 1.001) * (T * T);
       const double a2 = (8.008 * 8.008) * ((double)cut * (double)cut);
+      const float rt = scorer_tp_r(x, y, z);
 #pragma unroll
       for (int j = 0; j < NM; ++j) {
         const double rc2 = prm[P_RC2(NM) + j];
-        if (cut <= 8.0f && scorer_far(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
-                                      bv[3 * j + 1], bv[3 * j + 2], a2 * rc2, uT2))
-          continue;
-        scorer_features(x, y, z, vx, vy, vz, bx[3 * j], bx[3 * j + 1], bx[3 * j + 2], bv[3 * j],
-                        bv[3 * j + 1], bv[3 * j + 2], scorer_rc(rc2), T, f[j]);
+        const double X = bx[3 * j], Y = bx[3 * j + 1], Z = bx[3 * j + 2];
+        const double dx = X - x, dy = Y - y, dz = Z - z;
+        const double ux = bv[3 * j] - vx, uy = bv[3 * j + 1] - vy, uz = bv[3 * j + 2] - vz;
+        const bool far = cut <= 8.0f && scorer_far(dx, dy, dz, ux, uy, uz, a2 * rc2, uT2);
+        if (far) continue;
+        scorer_features_d(dx, dy, dz, ux, uy, uz, X, Y, Z, rt, scorer_rc(rc2), T, f[j]);
         if (!(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
       }
     };

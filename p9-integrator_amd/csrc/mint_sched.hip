@@ -173,9 +173,15 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     const int p = w % 2, B = w % 3, C = win[w];
     const double *wx = w == 0 ? c->d_mx : c->d_snap[B];  // window start state
     const double *wv = w == 0 ? c->d_mv : c->d_snap[B] + nb;
+    // ephemeris of window w (its end state is window w+1's snapshot, buffer
+    // (w+1) mod 3, last read by the queue head of window w-2); enqueued first
+    // so that its few waves are placed before the queue head fills the CUs
+    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
+    if ((rc = window_eph(c, sp, C, wx, wv, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
+    if ((rc = record(c->ev_eph, w, sp))) return rc;
     // queue head of window w, from the window-start state, beside the
-    // window's ephemeris; enqueued first, as it is the longer of the two
-    // (window 0's queue was cleared by the previous call or the upload)
+    // window's ephemeris (window 0's queue was cleared by the previous call or
+    // the upload)
     if (w > 0) {
       window_start_kernel<<<(unsigned)((S + 255) / 256), 256, 0, sm>>>(ctl[p], need[p], S, nullptr, nullptr,
                                                                        nullptr, nullptr, 0);
@@ -186,11 +192,6 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
                                         slist_head[p], stats),
                      "queue head launch")))
       return rc;
-    // ephemeris of window w (its end state is window w+1's snapshot, buffer
-    // (w+1) mod 3, last read by the queue head of window w-2)
-    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
-    if ((rc = window_eph(c, sp, C, wx, wv, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
-    if ((rc = record(c->ev_eph, w, sp))) return rc;
     if ((rc = wait(sm, c->ev_eph, w))) return rc;
     // queue stream: tails of window w-1, then the head of window w
     if ((rc = record(c->ev_fork, w, sm)) || (rc = wait(sq, c->ev_fork, w))) return rc;
