@@ -133,6 +133,16 @@ __device__ __forceinline__ double dot3(double ax, double ay, double az, double b
   return fma(ax, bx, fma(ay, by, az * bz));
 }
 
+// Coefficients of the series bands above |z| = 1e-5, highest degree first:
+// c2's (1/12!, ..., 1/6! with alternating signs), c3's (1/13!, ..., 1/7!),
+// then c2's and c3's degree-1 terms -1/4! and -1/5!.  Read from constant
+// memory where a band needs them, so that kernels looping over drifts do not
+// hold them in vector registers.
+static __constant__ double k_stumpff_hi[12] = {
+    1.1470745597729725e-11, -2.08767569878681e-09, 2.755731922398589e-07, -2.48015873015873e-05,
+    0.001388888888888889,   7.647163731819816e-13, -1.6059043836821613e-10, 2.505210838544172e-08,
+    -2.7557319223985893e-06, 0.0001984126984126984, -0.041666666666666664, -0.008333333333333333};
+
 // c0..c3 by the series alone, for |z| <= 0.1.
 __device__ __forceinline__ void stumpff_series(double z, double &c0, double &c1, double &c2,
                                                double &c3) {
@@ -144,25 +154,24 @@ __device__ __forceinline__ void stumpff_series(double z, double &c0, double &c1,
   double t2 = fma(0.001388888888888889, z, -0.041666666666666664);
   double t3 = fma(0.0001984126984126984, z, -0.008333333333333333);
   if (!(az <= 1e-5)) {
+    // the table's address laundered here, so its loads stay in this branch
+    const double *tb = k_stumpff_hi;
+    asm volatile("" : "+s"(tb));
     if (az <= 1e-3) {
-      t2 = -2.48015873015873e-05;
-      t3 = -2.7557319223985893e-06;
-      t2 = fma(t2, z, 0.001388888888888889);
-      t3 = fma(t3, z, 0.0001984126984126984);
+      t2 = fma(tb[3], z, tb[4]);
+      t3 = fma(tb[8], z, tb[9]);
     } else {
-      t2 = 1.1470745597729725e-11;
-      t2 = fma(t2, z, -2.08767569878681e-09);
-      t2 = fma(t2, z, 2.755731922398589e-07);
-      t2 = fma(t2, z, -2.48015873015873e-05);
-      t2 = fma(t2, z, 0.001388888888888889);
-      t3 = 7.647163731819816e-13;
-      t3 = fma(t3, z, -1.6059043836821613e-10);
-      t3 = fma(t3, z, 2.505210838544172e-08);
-      t3 = fma(t3, z, -2.7557319223985893e-06);
-      t3 = fma(t3, z, 0.0001984126984126984);
+      t2 = fma(tb[0], z, tb[1]);
+      t2 = fma(t2, z, tb[2]);
+      t2 = fma(t2, z, tb[3]);
+      t2 = fma(t2, z, tb[4]);
+      t3 = fma(tb[5], z, tb[6]);
+      t3 = fma(t3, z, tb[7]);
+      t3 = fma(t3, z, tb[8]);
+      t3 = fma(t3, z, tb[9]);
     }
-    t2 = fma(t2, z, -0.041666666666666664);
-    t3 = fma(t3, z, -0.008333333333333333);
+    t2 = fma(t2, z, tb[10]);
+    t3 = fma(t3, z, tb[11]);
   }
   t2 = fma(t2, z, 0.5);
   t3 = fma(t3, z, 0.16666666666666666);
