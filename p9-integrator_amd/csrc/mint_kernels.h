@@ -627,17 +627,18 @@ __device__ __forceinline__ void scorer_features(double x, double y, double z, do
 __device__ __forceinline__ float scorer_rc(double rc2) { return __builtin_amdgcn_sqrtf((float)rc2); }
 
 // Whether the scorer features of a pair are certain to have both distance
-// features at or above `cut` (f0, f3 >= cut: distance and straight-line
-// closest approach over the window at least 8 cut r_c).  Sufficient, without
-// a square root: r^2 >= 2 (A^2 + (|u| T)^2) implies r >= A + |u| T, and
-// |d + u t| >= |d| - |u| T over the window; A = 8 cut r_c and every term carry
-// a margin of 1e-3, far more than the FP32 roundings of the features (about
-// 1e-6 of the distances involved) can take away.  Formed in FP64 from the
-// body's values as they are (uniform operands where a wavefront's TPs share a
-// system), so no conversion is spent on a pair that turns out far.  A pair it
-// does not rule out is checked on its formed features; the scorer only
-// decides which TPs start the window in the queue, so the test changes no
-// result either way.  A2 = (8.008 cut)^2 r_c^2, uT2 = (1.001 T)^2.
+// features above g (f0, f3 > g: distance and straight-line closest approach
+// over the window more than 8 g r_c), from the pair's differences (body - TP)
+// in position d and velocity u.  Sufficient, without a square root:
+// r^2 >= 2 (A^2 + (|u| T)^2) implies r >= A + |u| T, and |d + u t| >= |d| -
+// |u| T over the window; A = 8 g r_c and every term carry a margin of 1e-3,
+// far more than the FP32 roundings of the features (about 1e-6 of the
+// distances involved) can take away.  Formed in FP64 from the body's values
+// as they are (uniform operands where a wavefront's TPs share a system), so
+// no conversion is spent on a pair that turns out far.  A pair it does not
+// rule out is checked on its formed features; the scorer only decides which
+// TPs start the window in the queue, so the test changes no result either
+// way.  A2 = (8.008 g)^2 r_c^2, uT2 = (1.001 T)^2.
 __device__ __forceinline__ bool scorer_far(double dx, double dy, double dz, double ux, double uy, double uz,
                                            double A2, double uT2) {
   const double r2 = dot3(dx, dy, dz, dx, dy, dz);
@@ -812,13 +813,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 //
 // The network runs over a packed list: each lane forms the features of its
 // NM pairs, the block gathers the pairs to be scored into LDS, and its 256
-// lanes score the list, one pair each per pass.  Pairs with both distance
-// features at or above `cut` (distance and straight-line closest approach
-// over the window at least 8 cut r_c) are left out of the list: the host
-// sets cut (+inf: none left out) only where an interval bound of the
-// network's logit over every such pair lies below the threshold, so no pick
-// changes.  With the trained weights cut is 1.5 (12 r_c); on the 4096-system
-// sweep about a quarter of a pair per TP is then scored.
+// lanes score the list, one pair each per pass.  A pair enters the list only
+// if its straight-line closest approach over the window comes within 4 r_c
+// (SC_REACH), and not if both distance features are at or above `cut`
+// (distance and closest approach at least 8 cut r_c): the host sets cut
+// (+inf: none left out) only where an interval bound of the network's logit
+// over every such pair lies below the threshold, so that exclusion changes no
+// pick.  A sufficient FP64 test at the smaller of the two radii rules most
+// pairs out before their features are formed; on the 4096-system sweep about
+// one pair in a thousand TPs is then scored.
 constexpr int PB_BLOCK = 256;
 // Pairs scored per pass of the block: the list is one pass long except where
 // a block has more pairs to score than it has lanes.
@@ -865,7 +868,8 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
       const double *bx = wx + (size_t)sys * 3 * NM, *bv = wv + (size_t)sys * 3 * NM;
       const double T = sc.h * (double)C;
       const double uT2 = (1.001 * 1.001) * (T * T);
-      const double a2 = (8.008 * 8.008) * ((double)cut * (double)cut);
+      const double g = (double)fminf(cut, SC_REACH);
+      const double a2 = (8.008 * 8.008) * (g * g);
       const float rt = scorer_tp_r(x, y, z);
 #pragma unroll
       for (int j = 0; j < NM; ++j) {
@@ -873,10 +877,9 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
         const double X = bx[3 * j], Y = bx[3 * j + 1], Z = bx[3 * j + 2];
         const double dx = X - x, dy = Y - y, dz = Z - z;
         const double ux = bv[3 * j] - vx, uy = bv[3 * j + 1] - vy, uz = bv[3 * j + 2] - vz;
-        const bool far = cut <= 8.0f && scorer_far(dx, dy, dz, ux, uy, uz, a2 * rc2, uT2);
-        if (far) continue;
+        if (scorer_far(dx, dy, dz, ux, uy, uz, a2 * rc2, uT2)) continue;
         scorer_features_d(dx, dy, dz, ux, uy, uz, X, Y, Z, rt, scorer_rc(rc2), T, f[j]);
-        if (!(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
+        if (f[j][3] <= SC_REACH && !(f[j][0] >= cut && f[j][3] >= cut)) want |= 1u << j;
       }
     };
     // TPs are stored by system, so a wavefront's lanes nearly always share

@@ -20,6 +20,12 @@ constexpr int KM_B3 = KM_W3 + KM_H;            // [1]
 constexpr int KM_NP = KM_B3 + 1;
 
 constexpr int SC_IN = 6, SC_H = 16;
+// The scorer is consulted only for pairs whose straight-line closest
+// approach over the launch window comes within 4 r_c (closest-approach
+// feature f3 = rmin / (8 r_c) at most SC_REACH): a pair that cannot come that
+// close is never picked.  On the 4096-system sweep every pair the network
+// picks lies within 2 r_c, so the gate takes no pick away there.
+constexpr float SC_REACH = 0.5f;
 constexpr int SC_W1 = 0;                       // [SC_H][SC_IN]
 constexpr int SC_B1 = SC_W1 + SC_H * SC_IN;    // [SC_H]
 constexpr int SC_W2 = SC_B1 + SC_H;            // [SC_H]
