@@ -39,13 +39,12 @@ FP64_PEAK = 78.6e12   # MI355X dense FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12     # B/s
 N_M = 5
 # HBM bytes per tp_main_kernel launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (calibrated by a 1 GiB copy: FETCH x2.0, WRITE x1.0), per launch shape
-# (block, TPs per lane, window); profiles/tp_main_kernel_p9sweep4096.md
-# and window length of the launch: {(block, TPs per lane, steps per launch): (bytes, source)}
-MEASURED_TRAFFIC = {(256, 1, 128): (1038.3e6, "profiles/tp_main_kernel_p9sweep4096.md (256 steps, 2 launches)"),
-                    (256, 1, 20): (783.8e6, "profiles/tp_main_kernel_p9sweep4096_20steps.md (20-step calls)"),
-                    (512, 1, 128): (1020.9e6, "profiles/tp_main_kernel_p9sweep4096_block512.md (256 steps, 2 launches)"),
-                    (512, 1, 20): (767.3e6, "profiles/tp_main_kernel_p9sweep4096_block512.md (20-step calls)")}
+# (calibrated by a 1 GiB copy: FETCH x2.0, WRITE x1.0), keyed by the launch's
+# (block, TPs per lane, steps per launch): {key: (bytes, source)}
+MEASURED_TRAFFIC = {(512, 1, 128): (807.5e6, "profiles/tp_main_kernel_p9sweep4096.md (256-step calls, 128-step launches)"),
+                    (512, 1, 20): (580.2e6, "profiles/tp_main_kernel_p9sweep4096_20steps.md (20-step calls)"),
+                    (256, 1, 128): (807.9e6, "profiles/tp_main_kernel_p9sweep4096.md (256-step calls, block 256)"),
+                    (256, 1, 20): (580.9e6, "profiles/tp_main_kernel_p9sweep4096_20steps.md (20-step calls, block 256)")}
 PLAIN_BYTES_PER_TP_STEP = 96  # SoA6 state read and written once per step
 
 
