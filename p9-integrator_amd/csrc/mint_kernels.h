@@ -812,7 +812,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 // finishing), the queue entry reads it when served.
 //
 // The network runs over a packed list: each lane forms the features of its
-// NM pairs, the block gathers the pairs to be scored into LDS, and its 256
+// NM pairs, the wavefront gathers the pairs to be scored into LDS, and its 64
 // lanes score the list, one pair each per pass.  A pair enters the list only
 // if its straight-line closest approach over the window comes within 4 r_c
 // (SC_REACH), and not if both distance features are at or above `cut`
@@ -823,9 +823,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
 // pairs out before their features are formed; on the 4096-system sweep about
 // one pair in a thousand TPs is then scored.
 constexpr int PB_BLOCK = 256;
-// Pairs scored per pass of the block: the list is one pass long except where
-// a block has more pairs to score than it has lanes.
-constexpr int PB_CAP = PB_BLOCK;
+
+// Orders a wavefront's LDS writes before its later LDS reads (the lanes of
+// one wavefront only: no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int NM>
 __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
@@ -834,12 +839,12 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     const float *__restrict__ scw, float thresh, float cut, unsigned char *__restrict__ carry,
     unsigned char *__restrict__ pre, QEntry *__restrict__ q, unsigned int *__restrict__ qcount,
     unsigned int *__restrict__ need, int *__restrict__ slist, unsigned long long *__restrict__ stats) {
-  // 6.8 KB of LDS per block (the packed list of one pass, each entry's owner
-  // lane, a pick flag per lane), so that LDS does not cap the blocks per CU
-  __shared__ float sh_f[PB_CAP][SC_IN];
-  __shared__ unsigned short sh_item[PB_CAP];
+  // Each wavefront packs and scores its own pairs in its own 64 rows: 6.6 KB
+  // of LDS per block (a pass of 64 entries per wavefront, each entry's owner
+  // lane, a pick flag per lane) and no workgroup barrier
+  __shared__ float sh_f[PB_BLOCK][SC_IN];
+  __shared__ unsigned char sh_item[PB_BLOCK];
   __shared__ unsigned char sh_go[PB_BLOCK];
-  __shared__ unsigned int sh_n;
   const int t = (int)threadIdx.x;
   const long long i = (long long)blockIdx.x * PB_BLOCK + t;
   unsigned int n_pre = 0;
@@ -856,8 +861,7 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
   }
   bool go = i < n && carried;
   if (go) carry[i] = 0;
-  if (scw != nullptr) {  // uniform: every lane takes part in the block's phases
-    if (t == 0) sh_n = 0u;
+  if (scw != nullptr) {  // uniform: every lane takes part in its wavefront's phases
     sh_go[t] = 0;
     float f[NM][SC_IN];
     unsigned int want = 0u;
@@ -893,40 +897,33 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
     } else if (mine) {
       pairs(s);
     }
-    // slots: a wavefront prefix sum of the lanes' counts, one LDS add per wavefront
-    const unsigned int cnt = (unsigned int)__builtin_popcount(want);
-    unsigned int incl = cnt;
+    // slots in the wavefront's list, body by body: pair (lane, j) goes after
+    // the pairs of bodies < j and of lower lanes with body j (ballots)
+    const unsigned int lane = (unsigned int)t & 63u, wb = (unsigned int)t & ~63u;
+    unsigned int slot[NM], wtot = 0u;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const unsigned int o = __shfl_up(incl, off, 64);
-      if ((t & 63) >= off) incl += o;
+    for (int j = 0; j < NM; ++j) {
+      const unsigned long long bj = __ballot((want >> j) & 1u);
+      slot[j] = wtot + __builtin_amdgcn_mbcnt_hi((unsigned int)(bj >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned int)bj, 0u));
+      wtot += (unsigned int)__popcll(bj);
     }
-    const unsigned int wtot = __shfl(incl, 63, 64);
-    __syncthreads();  // sh_n and sh_go initialised
-    unsigned int wbase = 0u;
-    if ((t & 63) == 63) wbase = atomicAdd(&sh_n, wtot);
-    wbase = __shfl(wbase, 63, 64);
-    const unsigned int slot0 = wbase + incl - cnt;
-    __syncthreads();
-    const unsigned int ni = sh_n;
-    // passes over the list, PB_CAP entries each: a lane writes the entries
-    // of its pairs that fall in the pass, lane e scores entry e, and a pair
-    // above the threshold flags its owner (a pair above it is what the
-    // largest logit above it means; a NaN logit flags nothing either way)
-    for (unsigned int p0 = 0; p0 < ni; p0 += PB_CAP) {
-      unsigned int sl = slot0;
+    // passes over the list, 64 entries each (nearly always none or one): a
+    // lane writes the entries of its pairs that fall in the pass, lane e
+    // scores entry e, and a pair above the threshold flags its owner (a pair
+    // above it is what the largest logit above it means; a NaN logit flags
+    // nothing either way)
+    for (unsigned int p0 = 0; p0 < wtot; p0 += 64u) {
 #pragma unroll
       for (int j = 0; j < NM; ++j) {
-        if (!((want >> j) & 1u)) continue;
-        if (sl >= p0 && sl < p0 + PB_CAP) {
+        if (((want >> j) & 1u) && slot[j] >= p0 && slot[j] < p0 + 64u) {
 #pragma unroll
-          for (int k = 0; k < SC_IN; ++k) sh_f[sl - p0][k] = f[j][k];
-          sh_item[sl - p0] = (unsigned short)t;
+          for (int k = 0; k < SC_IN; ++k) sh_f[wb + slot[j] - p0][k] = f[j][k];
+          sh_item[wb + slot[j] - p0] = (unsigned char)lane;
         }
-        ++sl;
       }
-      __syncthreads();
-      if ((unsigned int)t < ni - p0) {
+      wave_lds_sync();
+      if (lane < wtot - p0) {
         float g[SC_IN];
 #pragma unroll
         for (int k = 0; k < SC_IN; ++k) g[k] = sh_f[t][k];
@@ -940,9 +937,9 @@ __global__ void __launch_bounds__(PB_BLOCK) tp_prebucket_kernel(
           for (int k = 0; k < SC_IN; ++k) a = fmaf(scw[SC_W1 + u * SC_IN + k], g[k], a);
           out = fmaf(scw[SC_W2 + u], nn_tanh(a), out);
         }
-        if (out > thresh) sh_go[sh_item[t]] = 1;
+        if (out > thresh) sh_go[wb + sh_item[t]] = 1;
       }
-      __syncthreads();
+      wave_lds_sync();
     }
     if (mine) {
       go = sh_go[t] != 0;
