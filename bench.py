@@ -393,10 +393,15 @@ def main():
         g = make_ctx(form, guess, scorer, stats=kt, chunk_steps=T.window())
     else:
         g = make_ctx(form, guess, scorer, stats=kt)
+    tuned = None
     if form == mint.FORM_BATCHED and not shape:
         g.upload(st)
-        shape = tuple(g.autotune(args.tune_cache))
-        log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]}")
+        tuned = tuple(g.autotune(args.tune_cache))
+        # the shape a call of --steps steps uses (calls of one launch window
+        # have their own tuned block and TPs per lane)
+        shape = tuple(g.launch_shape(args.steps))
+        log(f"bench: launch shape block={shape[0]} tps/lane={shape[1]} window={shape[2]} for {args.steps}-step "
+            f"calls (longer calls: {tuned})")
     wall, ev_s, E0, E1, tm, calls_ms = timed_run(g, st, args.steps, args.warmup, args.repeats)
     status = g.status()
     fin = g.download()
@@ -514,7 +519,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "p9_sweep_4096", "systems": st.n_systems, "tp_total": ntp, "massive_per_system": 5,
                    "dt_yr": 0.5, "form": args.form, "kepler_guess": int(guess), "scorer": scorer,
-                   "launch_shape": list(shape) if shape else None},
+                   "launch_shape": list(shape) if shape else None,
+                   "launch_shape_multiwindow_calls": list(tuned) if tuned else None},
         "roofline": roof,
         "cpu_baseline": base,
         "forms_tp_steps_per_s": forms,

@@ -256,12 +256,19 @@ int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const doub
                                double *ecc, double *sma);
 
 /* Autotuner (batched form): time the candidate launch shapes on the uploaded
- * batch (each once untimed, then a timed 256-step probe), keep the fastest,
- * and store it in cache_path (NULL = no file).  A shape parameter set in the
- * configuration (block_size, tps_per_lane, chunk_steps > 0) is held at that
- * value.  The state on the device is left unchanged.
- * best[3] = {block, tps/lane, chunk}. */
+ * batch (each once untimed, then the best of two timed 256-step probes), keep
+ * the fastest, then the same for block and TPs per lane on calls of one
+ * launch window (a probe of up to 32 steps; only where neither is set in the
+ * configuration), and store both in cache_path (NULL = no file).  A shape
+ * parameter set in the configuration (block_size, tps_per_lane, chunk_steps
+ * > 0) is held at that value.  The state on the device is left unchanged.
+ * best[3] = {block, tps/lane, chunk} of calls of more than one window. */
 int mint_autotune(void *ctx, const char *cache_path, int32_t *best);
+
+/* The launch shape a batched call of nsteps steps uses: shape[3] = {block,
+ * tps/lane, chunk}.  No counterpart in the reference (its launch shape is
+ * fixed, src/mercurius.cu:45-65); for reporting and benchmarks. */
+int mint_launch_shape(void *ctx, int64_t nsteps, int32_t *shape);
 
 /* Learned parts (FP32).  Weights live in the context.
  * Kepler warm-start MLP: 4 -> 32 -> 32 -> 1, tanh.  Scorer: 6 -> 16 -> 1, tanh/logistic.

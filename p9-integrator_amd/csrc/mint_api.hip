@@ -86,6 +86,9 @@ void Ctx::free_device() {
   many = false;
   rec_steps_alloc = 0;
   tiles_key = -1;
+  ntiles = 0;
+  for (auto &ts : tilesets) ts = TileSet{};
+  tiles_next = 0;
   uploaded = false;
 }
 
@@ -194,6 +197,14 @@ TPArrays Ctx::tp_arrays() const {
 int Ctx::ensure_tiles(int block, int P) {
   const long long key = (long long)block * 16 + P;
   if (key == tiles_key) return MINT_OK;
+  for (int k = 0; k < 2; ++k)
+    if (tilesets[k].key == key) {
+      d_tiles = tilesets[k].d;
+      ntiles = tilesets[k].n;
+      tiles_key = key;
+      tiles_next = 1 - k;
+      return MINT_OK;
+    }
   std::vector<Tile> tl;
   const long long per = (long long)block * P;
   for (int s = 0; s < S; ++s) {
@@ -205,20 +216,29 @@ int Ctx::ensure_tiles(int block, int P) {
       tl.push_back(t);
     }
   }
-  if (d_tiles) {
-    hipFree(d_tiles);
-    allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)d_tiles));
-    d_tiles = nullptr;
+  // replace the set used least recently; graphs captured with it hold its
+  // pointer, so they go
+  TileSet &ts = tilesets[tiles_next];
+  if (ts.d) {
+    drop_graphs();
+    hipFree(ts.d);
+    allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)ts.d));
+    ts = TileSet{};
   }
-  ntiles = (int)tl.size();
-  int rc = dalloc((void **)&d_tiles, tl.size() * sizeof(Tile), "tiles");
+  d_tiles = nullptr;
+  tiles_key = -1;
+  int rc = dalloc((void **)&ts.d, tl.size() * sizeof(Tile), "tiles");
   if (rc) return rc;
+  ts.n = (int)tl.size();
   if (!tl.empty()) {
-    rc = hip(hipMemcpy(d_tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice), "tiles H2D");
+    rc = hip(hipMemcpy(ts.d, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice), "tiles H2D");
     if (rc) return rc;
   }
+  ts.key = key;
+  d_tiles = ts.d;
+  ntiles = ts.n;
   tiles_key = key;
-  drop_graphs();
+  tiles_next = 1 - tiles_next;
   return MINT_OK;
 }
 

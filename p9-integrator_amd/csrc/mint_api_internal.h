@@ -85,6 +85,16 @@ struct Ctx {
   Tile *d_tiles = nullptr;
   int ntiles = 0;
   long long tiles_key = -1;
+  // tile lists of the two most recent (block, TPs per lane) shapes, so that
+  // calls alternating between two shapes keep both lists (and the graphs that
+  // hold them); d_tiles, ntiles and tiles_key are the current one's
+  struct TileSet {
+    long long key = -1;
+    Tile *d = nullptr;
+    int n = 0;
+  };
+  TileSet tilesets[2];
+  int tiles_next = 0;  // the set a new shape replaces
   QEntry *d_q = nullptr, *d_q2 = nullptr;  // BS queues of even / odd windows
   // [2][8] per window parity: [0] entries of the queue head, [1] systems
   // flagged 1, [3] systems flagged 2, [4] mid-window deferrals (queued after
@@ -107,8 +117,10 @@ struct Ctx {
   float sc_sat_bound = INFINITY, sc_cut = INFINITY;
   bool scorer_skip_on() const { return scorer_skip_mode == 1 && !w_sc.empty() && sc_cut <= 8.0f; }
 
-  // launch shape chosen for the batched form
-  int tuned_block = 0, tuned_P = 0, tuned_C = 0;
+  // launch shape chosen for the batched form (calls of more than one launch
+  // window), and the block and TPs per lane chosen for calls of one window
+  // (0: the same as for longer calls)
+  int tuned_block = 0, tuned_P = 0, tuned_C = 0, tuned_block1 = 0, tuned_P1 = 0;
 
   std::vector<GraphEntry> graphs;
   mint_timing timing{};

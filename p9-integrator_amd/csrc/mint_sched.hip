@@ -30,11 +30,19 @@ struct Shape {
   int block, P, C;
 };
 
-Shape batched_shape(const Ctx *c) {
+// The launch shape of an nsteps call: the configuration's held values, else
+// the tuned ones.  A call of one launch window has its own tuned block and
+// TPs per lane (where neither is held): with no window after it, its queue
+// work only has to fit beside one main kernel, which favours smaller blocks.
+Shape batched_shape(const Ctx *c, int64_t nsteps) {
   Shape s;
   s.block = c->cfg.block_size > 0 ? c->cfg.block_size : (c->tuned_block > 0 ? c->tuned_block : DEF_BLOCK);
   s.P = c->cfg.tps_per_lane > 0 ? c->cfg.tps_per_lane : (c->tuned_P > 0 ? c->tuned_P : DEF_P);
   s.C = c->cfg.chunk_steps > 0 ? c->cfg.chunk_steps : (c->tuned_C > 0 ? c->tuned_C : DEF_C);
+  if (nsteps > 0 && nsteps <= s.C && c->cfg.block_size <= 0 && c->cfg.tps_per_lane <= 0 && c->tuned_block1 > 0) {
+    s.block = c->tuned_block1;
+    s.P = c->tuned_P1;
+  }
   return s;
 }
 
@@ -276,7 +284,7 @@ int graph_for(Ctx *c, const Shape &sh, int64_t nsteps, hipGraphExec_t *out, int 
 // Shape checks and buffers of an nsteps batched call at the context's launch
 // shape (all allocation happens here, before any capture).
 int batched_setup(Ctx *c, int64_t nsteps, Shape *out) {
-  const Shape sh = batched_shape(c);
+  const Shape sh = batched_shape(c, nsteps);
   if (!(sh.P == 1 || sh.P == 2 || sh.P == 4)) return c->fail(MINT_EINVAL, "tps_per_lane must be 1, 2 or 4");
   if (sh.block < 64 || sh.block > 512 || (sh.block & 63)) return c->fail(MINT_EINVAL, "block_size must be a multiple of 64 in [64,512]");
   if (sh.C < 1 || sh.C > 4096) return c->fail(MINT_EINVAL, "chunk_steps must be in [1,4096]");
@@ -443,8 +451,19 @@ int mint_prepare(void *ctx, int64_t nsteps) {
 }
 
 // ---------------------------------------------------------------------------
+int mint_launch_shape(void *ctx, int64_t nsteps, int32_t *shape) {
+  Ctx *c = (Ctx *)ctx;
+  if (!c || !shape || nsteps < 0) return MINT_EINVAL;
+  const Shape sh = batched_shape(c, nsteps);
+  shape[0] = sh.block;
+  shape[1] = sh.P;
+  shape[2] = sh.C;
+  return MINT_OK;
+}
+
 // Autotuner: time each candidate (block, TPs/lane, window) on the uploaded
-// batch for a probe of 128 steps, restoring the device state after each.
+// batch for a probe of 256 steps, then block and TPs/lane for calls of one
+// window, restoring the device state after each run.
 int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   Ctx *c = (Ctx *)ctx;
   if (!c) return MINT_EINVAL;
@@ -469,11 +488,14 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
     while (std::getline(in, line)) {
       std::istringstream is(line);
       std::string k;
-      int b, p, cc;
+      int b, p, cc, b1 = 0, p1 = 0;
       if (is >> k >> b >> p >> cc && k == key) {
+        if (!(is >> b1 >> p1)) b1 = p1 = 0;  // no one-window shape on the line
         c->tuned_block = b;
         c->tuned_P = p;
         c->tuned_C = cc;
+        c->tuned_block1 = b1;
+        c->tuned_P1 = p1;
         if (best) {
           best[0] = b;
           best[1] = p;
@@ -517,36 +539,61 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   if (saved.block_size > 0) blocks = {saved.block_size};
   if (saved.tps_per_lane > 0) Ps = {saved.tps_per_lane};
   if (saved.chunk_steps > 0) Cs = {saved.chunk_steps};
-  // two or more windows, so the probe includes the pipeline's steady state
-  const int probe = 256;
-  double best_ms = 1e300;
-  Shape bs{blocks[0], Ps[0], Cs[0]};
+  // Two passes over the candidates, each candidate's best time kept: a
+  // single pass in a fixed order favoured the candidates timed last, once the
+  // clock had settled under the load.
   int rc = MINT_OK;
   c->cfg.form = MINT_FORM_BATCHED;
   c->cfg.collect_stats = 0;
-  for (int b : blocks)
-    for (int p : Ps)
-      for (int C : Cs) {
-        c->cfg.block_size = b;
-        c->cfg.tps_per_lane = p;
-        c->cfg.chunk_steps = C;
-        // untimed first run (buffers, tiles and the graph for this shape), then the probe
-        restore();
-        if ((rc = step_impl(c, probe))) break;
+  auto tune = [&](const std::vector<Shape> &cand, int probe, Shape *out) -> int {
+    std::vector<double> best_of(cand.size(), 1e300);
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t k = 0; k < cand.size(); ++k) {
+        c->cfg.block_size = cand[k].block;
+        c->cfg.tps_per_lane = cand[k].P;
+        c->cfg.chunk_steps = cand[k].C;
+        int r;
+        // first pass: an untimed run first (buffers, tiles and the graph for this shape)
+        if (pass == 0) {
+          restore();
+          if ((r = step_impl(c, probe))) return r;
+        }
         restore();
         hipDeviceSynchronize();
         hipEventRecord(c->ev0, c->stream);
-        rc = step_impl(c, probe);
+        r = step_impl(c, probe);
         hipEventRecord(c->ev1, c->stream);
         hipEventSynchronize(c->ev1);
-        if (rc) break;
+        if (r) return r;
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        if (ms < best_ms) {
-          best_ms = ms;
-          bs = Shape{b, p, C};
-        }
+        best_of[k] = std::min(best_of[k], (double)ms);
       }
+    double best_ms = 1e300;
+    for (size_t k = 0; k < cand.size(); ++k)
+      if (best_of[k] < best_ms) {
+        best_ms = best_of[k];
+        *out = cand[k];
+      }
+    return MINT_OK;
+  };
+  // calls of more than one window: a 256-step probe (two or more windows, so
+  // it includes the pipeline's steady state)
+  std::vector<Shape> cand;
+  for (int b : blocks)
+    for (int p : Ps)
+      for (int C : Cs) cand.push_back(Shape{b, p, C});
+  Shape bs = cand[0];
+  rc = tune(cand, 256, &bs);
+  // calls of one window (where block and TPs per lane are both free): a
+  // probe of one window of at most 32 steps at the chosen window length
+  Shape b1{0, 0, bs.C};
+  if (rc == MINT_OK && saved.block_size <= 0 && saved.tps_per_lane <= 0) {
+    std::vector<Shape> cand1;
+    for (int b : blocks)
+      for (int p : Ps) cand1.push_back(Shape{b, p, bs.C});
+    rc = tune(cand1, std::min(bs.C, 32), &b1);
+  }
   c->cfg = saved;
   restore();
   hipDeviceSynchronize();
@@ -556,6 +603,8 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   c->tuned_block = bs.block;
   c->tuned_P = bs.P;
   c->tuned_C = bs.C;
+  c->tuned_block1 = b1.block;
+  c->tuned_P1 = b1.P;
   if (best) {
     best[0] = bs.block;
     best[1] = bs.P;
@@ -563,7 +612,7 @@ int mint_autotune(void *ctx, const char *cache_path, int32_t *best) {
   }
   if (cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    if (out) out << key << " " << bs.block << " " << bs.P << " " << bs.C << "\n";
+    if (out) out << key << " " << bs.block << " " << bs.P << " " << bs.C << " " << b1.block << " " << b1.P << "\n";
   }
   hipMemset(c->d_stats, 0, S_COUNT * sizeof(unsigned long long));
   return MINT_OK;

@@ -119,6 +119,7 @@ _SIGS = {
     "mint_elements_to_cartesian": (C.c_int, [C.c_int64, C.c_double] + [_DP] * 8),
     "mint_cartesian_to_elements": (C.c_int, [C.c_int64, C.c_double] + [_DP] * 8),
     "mint_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
+    "mint_launch_shape": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]),
     "mint_trace_kepler": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float),
                                     C.POINTER(C.c_float), C.POINTER(C.c_int64)]),
     "mint_trace_encounter": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float),
@@ -363,6 +364,12 @@ class Mint:
         best = (C.c_int32 * 3)()
         self._chk(self.L.mint_autotune(self.h, cache_path.encode() if cache_path else None, best), "mint_autotune")
         return tuple(best)
+
+    def launch_shape(self, nsteps):
+        """(block, TPs per lane, window) that a batched call of nsteps steps uses."""
+        sh = (C.c_int32 * 3)()
+        self._chk(self.L.mint_launch_shape(self.h, int(nsteps), sh), "mint_launch_shape")
+        return tuple(sh)
 
     # learned parts -------------------------------------------------------
     def trace_kepler(self, nsteps, cap):
