@@ -96,3 +96,39 @@ def test_main_kernel_timing_mode_changes_nothing_and_times_every_window(use_grap
     assert t["n_tp_main"] == 4          # windows of 32, 32, 32, 4 steps
     assert 0.0 < t["ms_tp_main"] <= t["ms_total"]
     assert tms[0]["n_tp_main"] == 0
+
+
+def test_calls_of_mixed_lengths_forms_and_uploads_keep_the_queues_clear():
+    """A batched call leaves the resident massive state and its first
+    window's queue as the next call expects them (the call's last kernel copies
+    the end state back and clears that queue): calls of one window, of
+    several windows with carried deferrals, flat calls between them and a
+    re-upload, at the tuned launch shapes (calls of one window and longer
+    calls may each have their own, and the context keeps both tile lists), on
+    an encounter-heavy system (mid-window deferrals in every window), all
+    bitwise equal to the oracle over the same steps."""
+    from gpu_cases import encounter_system
+
+    st = encounter_system(n_tp=300, seed=5)
+    print("queue hygiene: encounter cloud, calls 7, 40, flat 5, 16, 33 (re-upload), 3", flush=True)
+    with mint.Mint(dt=0.5, form=mint.FORM_BATCHED, chunk_steps=16, collect_stats=1) as g:
+        g.upload(st)
+        g.autotune(None)
+        print(f"shapes: one window {g.launch_shape(7)}, longer {g.launch_shape(40)}", flush=True)
+        g.step(7)
+        g.step(40)                       # three windows, carried tails
+        g.set_mode(mint.FORM_FLAT, mint.GUESS_SERIES, 0)
+        g.step(5)
+        g.set_mode(mint.FORM_BATCHED, mint.GUESS_SERIES, 0)
+        g.step(16)
+        mid = g.download()
+        deferred = g.stats()["deferred"]
+        g.upload(st)                     # same shape: buffers and graphs kept
+        g.step(33)
+        g.step(3)
+        end = g.download()
+    ref1, _, _ = O.run(st, 68, 0.5, guess=1)
+    ref2, _, _ = O.run(st, 36, 0.5, guess=1)
+    assert deferred > 0
+    _same(mid, ref1)
+    _same(end, ref2)
