@@ -286,7 +286,7 @@ def bench_embryos(args):
     t = gp.timing()
     s = gp.stats()
     gp.close()
-    ms = {k: t[f"ms_mb_{k}"] / max(t[f"n_mb_{k}"], 1) for k in ("open", "drift", "close")}
+    ms = {k: t[f"ms_mb_{k}"] / max(t[f"n_mb_{k}"], 1) for k in ("drift", "close")}
     F = flops_per_kick_pass(n)
     achieved = F / (ms["close"] / 1e3)
     roof = {"bound": "fp64", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",

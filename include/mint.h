@@ -139,7 +139,9 @@ typedef struct mint_timing {
   double ms_tp_main;    int64_t n_tp_main;
   double ms_encounter;  int64_t n_encounter;
   double ms_total;
-  /* many-body engine: encounter pairs and groups, drift, jump + kicks */
+  /* many-body engine: drift (with the momentum and jump before it), jump +
+     kicks (with the next step's pair check); ms_mb_open / n_mb_open stay 0:
+     the pair check has no launch of its own */
   double ms_mb_open;    int64_t n_mb_open;
   double ms_mb_drift;   int64_t n_mb_drift;
   double ms_mb_close;   int64_t n_mb_close;

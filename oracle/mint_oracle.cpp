@@ -562,6 +562,12 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt, uint32_t *hits = nu
   uint32_t st = 0;
   std::vector<V3> X(nm);
   for (int j = 0; j < nm; ++j) X[j] = s.mb[j].x;
+  // massive encounter groups from the separations at step start: jump 1
+  // translates every massive body by the same (h/2) Q1, so the pair
+  // separations it leaves are these up to the rounding of the translation
+  // (close_encounter_p, src/sim.cuh:589-613, tests them after the jump)
+  int64_t ov = 0;
+  const std::vector<std::vector<int>> groups = encounter_groups(s, ov);
   // 1. kick 1/2 (massive and TPs, positions at step start)
   const std::vector<V3> Am = accel_massive(X, s, p.eps2);
   for (Body &t : s.tp) {
@@ -587,7 +593,7 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt, uint32_t *hits = nu
     t.x.y = std::fma(hh, Q1.y, t.x.y);
     t.x.z = std::fma(hh, Q1.z, t.x.z);
   }
-  // 3. encounter flags on post-jump positions; 4. drift
+  // 3. TP encounter flags on post-jump positions; 4. drift
   const std::vector<Body> pert0 = s.mb;
   size_t ti = 0;
   for (Body &t : s.tp) {
@@ -615,8 +621,6 @@ uint32_t step(Sys &s, const Params &p, double M, Stats &stt, uint32_t *hits = nu
     }
     stt.tp_steps++;
   }
-  int64_t ov = 0;
-  const std::vector<std::vector<int>> groups = encounter_groups(s, ov);
   stt.overlaps += ov;
   std::vector<char> ingroup(nm, 0);
   int64_t nr = 0;

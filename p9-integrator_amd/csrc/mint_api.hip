@@ -80,7 +80,7 @@ void Ctx::free_device() {
   d_status = nullptr;
   d_stats = nullptr;
   d_nnw_kep = d_nnw_sc = nullptr;
-  d_bp = d_mxb = d_mvb = d_my = nullptr;
+  d_bp = d_mxb = d_mvb = d_mbmax = nullptr;
   d_mflag = d_medge = nullptr;
   d_mctl = nullptr;
   many = false;

@@ -130,10 +130,10 @@ struct Ctx {
   std::vector<double> rad;          // physical radii [S*NM] (zeros: none)
   double *d_bp = nullptr;           // [S][5][NM]: m, 0.1 r_c, 1/(0.9 r_c), r_c^2, R
   double *d_mxb = nullptr, *d_mvb = nullptr;  // post-drift x, v  [S][NM][3]
-  double *d_my = nullptr;           // post-jump positions (drift start) [S][NM][3]
+  double *d_mbmax = nullptr;        // [S][2]: largest r_c^2 and physical radius (pair-check bounds)
   int *d_mflag = nullptr;           // [S][NM] body is in an encounter pair this step
   int *d_medge = nullptr;           // [S][MB_EMAX][2] encounter pairs
-  unsigned int *d_mctl = nullptr;   // [S][4]: [0] pairs in the list this step
+  unsigned int *d_mctl = nullptr;   // [S][4]: [0], [1] pairs in the list of even, odd steps
   std::vector<hipEvent_t> tev;       // event pool of the per-kernel timing mode
   hipGraphExec_t mb_graph = nullptr;
   long long mb_graph_steps = 0;

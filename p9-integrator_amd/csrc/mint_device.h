@@ -959,35 +959,6 @@ __device__ __forceinline__ void momentum_q(const double *V, const double *__rest
   Q[2] = pz / M;
 }
 
-// Encounter graph of the massive bodies at X: adj[j] has bit k set when the
-// pair (j, k) is inside its changeover radius.  Also counts the pairs closer
-// than the sum of their physical radii.  Returns whether any edge exists.
-template <int NM>
-__device__ __forceinline__ bool encounter_graph(const double *X, const double *__restrict__ prm,
-                                                uint32_t *adj, unsigned int &overlaps) {
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < NM; ++j) adj[j] = 0u;
-#pragma unroll
-  for (int j = 0; j < NM; ++j) {
-#pragma unroll
-    for (int k = j + 1; k < NM; ++k) {
-      const double dx = X[3 * k + 0] - X[3 * j + 0];
-      const double dy = X[3 * k + 1] - X[3 * j + 1];
-      const double dz = X[3 * k + 2] - X[3 * j + 2];
-      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double sr = prm[P_RAD(NM) + j] + prm[P_RAD(NM) + k];
-      if (r2 < sr * sr) ++overlaps;
-      if (r2 < prm[P_PRC2(NM) + j * NM + k]) {
-        adj[j] |= 1u << k;
-        adj[k] |= 1u << j;
-        any = true;
-      }
-    }
-  }
-  return any;
-}
-
 // Close-encounter RHS of one encounter group (g members, body indices idx[],
 // state [g][6]): the Sun plus the (1-K)-weighted pulls of the other members.
 template <int NM>

@@ -183,7 +183,7 @@ __device__ __forceinline__ int tp_kepler(double &x, double &y, double &z, double
 // (the body count rounded up to a power of two), lane j holding body j, so a
 // step's five Kepler drifts, kicks and pair checks run side by side instead of
 // in one lane.  Each lane forms exactly what iteration j of the body loops of
-// accel_massive, momentum_q and encounter_graph forms, in the same order, with
+// accel_massive, momentum_q and the pair check forms, in the same order, with
 // the other bodies' positions and velocities taken from their lanes by
 // shuffles; the records are therefore the same bits.  Lanes beyond NM shadow
 // the last body (nothing of theirs is written).
@@ -354,26 +354,11 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
   uint32_t st = 0;
   for (int k = 0; k < C; ++k) {
     double *r = rec + ((size_t)s * C + k) * REC;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = fma(sc.hh, a[c], v[c]);
-    momentum_lane<NM>(v, base, lp, M, Q);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) x[c] = fma(sc.hh, Q[c], x[c]);
-    if (own) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        r[R_Y(NM) + 3 * jb + c] = x[c];
-        r[R_W(NM) + 3 * jb + c] = v[c];
-      }
-      if (jb == 0) {
-        r[R_Q1(NM) + 0] = Q[0];
-        r[R_Q1(NM) + 1] = Q[1];
-        r[R_Q1(NM) + 2] = Q[2];
-      }
-    }
-    // pair checks of encounter_graph: the separation of a pair is formed as
-    // (higher index) - (lower index) in both of its lanes; overlaps are
-    // counted by the lower index's lane
+    // pair checks of encounter_groups on the step-start positions (the
+    // separations jump 1 leaves, but for the rounding of its common
+    // translation): the separation of a pair is formed as (higher index) -
+    // (lower index) in both of its lanes; overlaps are counted by the lower
+    // index's lane
     uint32_t adj = 0u;
     unsigned int ov = 0;
 #pragma unroll
@@ -396,6 +381,23 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
     if (ov) {
       st |= ST_OVERLAP;
       if (stats) atomicAdd(stats + S_OVERLAP, (unsigned long long)ov);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = fma(sc.hh, a[c], v[c]);
+    momentum_lane<NM>(v, base, lp, M, Q);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c] = fma(sc.hh, Q[c], x[c]);
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[R_Y(NM) + 3 * jb + c] = x[c];
+        r[R_W(NM) + 3 * jb + c] = v[c];
+      }
+      if (jb == 0) {
+        r[R_Q1(NM) + 0] = Q[0];
+        r[R_Q1(NM) + 1] = Q[1];
+        r[R_Q1(NM) + 2] = Q[2];
+      }
     }
     bool ingrp = false;
     if (__builtin_amdgcn_ballot_w64(adj != 0u) != 0ull) {  // some system of this wave has a close pair

@@ -3,22 +3,29 @@
 //
 // It integrates the same massive-subsystem step as the lane-per-system
 // ephemeris kernel, operation for operation, so the two agree to the last bit
-// where both apply; what changes is the parallel shape.  One step is three
+// where both apply; what changes is the parallel shape.  One step is two
 // launches per system set (blockIdx.y = system):
 //
-//   mb_open_kernel   momentum after the opening kick, jump 1, and the exact
-//                    pair check of every pair (j < k) against its changeover
-//                    radius; pairs inside are appended to the step's pair list.
-//   mb_drift_kernel  Kepler drift of every body in no pair (one lane each);
-//                    meanwhile each group workgroup turns the pair list into
-//                    encounter groups (connected components, listed by
-//                    smallest member) in its LDS and drifts its share of them
-//                    by Bulirsch-Stoer (one wavefront per group: row groups
-//                    of lanes, one member per lane, run the Gragg rows side
-//                    by side; positions exchanged by wavefront shuffles).
+//   mb_drift_kernel  momentum after the opening kick and jump 1 (each
+//                    workgroup forms them), then the Kepler drift of every
+//                    body in no pair (one lane each); meanwhile each group
+//                    workgroup turns the step's pair list into encounter
+//                    groups (connected components, listed by smallest member)
+//                    in its LDS and drifts its share of them by Bulirsch-
+//                    Stoer (one wavefront per group: row groups of lanes, one
+//                    member per lane, run the Gragg rows side by side;
+//                    positions exchanged by wavefront shuffles).
 //   mb_close_kernel  momentum after the drift, jump 2, O(N^2) kick
 //                    accelerations and the closing kick (fused with the next
-//                    step's opening kick, which sees the same positions).
+//                    step's opening kick, which sees the same positions), and
+//                    the next step's exact pair check of every pair (j < k)
+//                    against its changeover radius on the same separations:
+//                    pairs inside are appended to the next step's pair list.
+//
+// The pair check is taken on a step's start positions, before jump 1: the
+// jump translates every body by the same (h/2) Q1, so it leaves the pair
+// separations as they are up to the rounding of the translation (the oracle
+// and the ephemeris kernel check the same separations).
 //
 // Reference: body_interaction_kick with every body massive (src/sim.cuh:327-382),
 // main_body_kinetic (src/sim.cuh:384-406), close_encounter_p
@@ -160,6 +167,36 @@ __device__ __forceinline__ int lds1(int k) { return k + (k >> 4); }      // scal
 // when the values go to LDS, once the momentum is known.
 constexpr int MB_PF = MB_MAXCS * MB_CHUNK / MB_BLOCK;
 
+// Chunk t's momentum sum over a staged window of nw bodies (sv in the lds3
+// layout, sm in the lds1 layout): from zero, in index order.  Slots past the
+// window (not staged) are read and their terms selected away, without
+// branches; eight bodies' loads at a time.
+__device__ __forceinline__ void chunk_momentum(const double *sv, const double *sm, int t, int nw, double &px,
+                                               double &py, double &pz) {
+  px = 0.0;
+  py = 0.0;
+  pz = 0.0;
+#pragma unroll
+  for (int h = 0; h < MB_CHUNK; h += MB_DIVG) {
+    double mm[MB_DIVG], vx[MB_DIVG], vy[MB_DIVG], vz[MB_DIVG];
+#pragma unroll
+    for (int q = 0; q < MB_DIVG; ++q) {
+      const int k = t * MB_CHUNK + h + q;
+      mm[q] = sm[lds1(k)];
+      vx[q] = sv[lds3(k) + 0];
+      vy[q] = sv[lds3(k) + 1];
+      vz[q] = sv[lds3(k) + 2];
+    }
+#pragma unroll
+    for (int q = 0; q < MB_DIVG; ++q) {
+      const bool in = t * MB_CHUNK + h + q < nw;
+      px = in ? fma(mm[q], vx[q], px) : px;
+      py = in ? fma(mm[q], vy[q], py) : py;
+      pz = in ? fma(mm[q], vz[q], pz) : pz;
+    }
+  }
+}
+
 // Q = (sum_j m_j v_j)/M by the whole workgroup.  Masses and velocities pass
 // through LDS a window of WB bodies at a time (sv in the lds3 layout, sm in
 // the lds1 layout; the first window comes from the caller's registers pm, pv,
@@ -198,28 +235,8 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
     }
     __syncthreads();
     if (t * MB_CHUNK < nw) {  // t < CS <= 64: wavefront 0
-      double px = 0.0, py = 0.0, pz = 0.0;
-#pragma unroll
-      for (int h = 0; h < MB_CHUNK; h += MB_DIVG) {  // eight bodies' loads at a time
-        // slots past the window (not staged) are read and their terms
-        // selected away, without branches
-        double mm[MB_DIVG], vx[MB_DIVG], vy[MB_DIVG], vz[MB_DIVG];
-#pragma unroll
-        for (int q = 0; q < MB_DIVG; ++q) {
-          const int k = t * MB_CHUNK + h + q;
-          mm[q] = sm[lds1(k)];
-          vx[q] = sv[lds3(k) + 0];
-          vy[q] = sv[lds3(k) + 1];
-          vz[q] = sv[lds3(k) + 2];
-        }
-#pragma unroll
-        for (int q = 0; q < MB_DIVG; ++q) {
-          const bool in = t * MB_CHUNK + h + q < nw;
-          px = in ? fma(mm[q], vx[q], px) : px;
-          py = in ? fma(mm[q], vy[q], py) : py;
-          pz = in ? fma(mm[q], vz[q], pz) : pz;
-        }
-      }
+      double px, py, pz;
+      chunk_momentum(sv, sm, t, nw, px, py, pz);
       if (pass == 0) {
         c0x = px;
         c0y = py;
@@ -247,23 +264,97 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
   __syncthreads();
 }
 
+// wg_momentum by one wavefront (a one-wave workgroup): the same windows, chunk
+// sums and fold, so the same Q, which every lane receives.  Each window's
+// velocities and masses are loaded coalesced (lane l takes bodies l, l + 64,
+// ...) into sv and sm (3 WB + CS and WB + CS doubles of LDS).  Called by all
+// 64 lanes; ends with a barrier, after which the caller may reuse the LDS.
+__device__ void wave_momentum(const double *__restrict__ V, const double *__restrict__ m, const MBShape &sh,
+                              double M, double *sv, double *sm, double &q0, double &q1, double &q2) {
+  const int t = (int)threadIdx.x, N = sh.N, WB = sh.CS * MB_CHUNK;
+  double c0x = 0.0, c0y = 0.0, c0z = 0.0, c1x = 0.0, c1y = 0.0, c1z = 0.0;
+  for (int b0 = 0, pass = 0; b0 < N; b0 += WB, ++pass) {
+    const int nw = min(WB, N - b0);
+    if (pass > 0) __syncthreads();  // the previous window has been read
+    // every load of the window in flight at once (a window holds at most
+    // MB_MAXCS x 16 = 1024 bodies, 16 a lane)
+    constexpr int U = MB_MAXCS * MB_CHUNK / 64;
+    double lv[U][3], lm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = t + 64 * u, kk = b0 + (k < nw ? k : 0);
+      lv[u][0] = V[3 * kk + 0];
+      lv[u][1] = V[3 * kk + 1];
+      lv[u][2] = V[3 * kk + 2];
+      lm[u] = m[kk];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = t + 64 * u;
+      if (k < nw) {
+        sv[lds3(k) + 0] = lv[u][0];
+        sv[lds3(k) + 1] = lv[u][1];
+        sv[lds3(k) + 2] = lv[u][2];
+        sm[lds1(k)] = lm[u];
+      }
+    }
+    __syncthreads();
+    if (t * MB_CHUNK < nw) {
+      double px, py, pz;
+      chunk_momentum(sv, sm, t, nw, px, py, pz);
+      if (pass == 0) {
+        c0x = px;
+        c0y = py;
+        c0z = pz;
+      } else {
+        c1x = px;
+        c1y = py;
+        c1z = pz;
+      }
+    }
+  }
+  if (sh.P > 64) {  // first level of the fold: chunk l + 64 (or +0)
+    c0x = c0x + c1x;
+    c0y = c0y + c1y;
+    c0z = c0z + c1z;
+  }
+  fold_lanes(c0x, c0y, c0z, sh.P < 64 ? sh.P : 64);
+  q0 = __shfl(c0x, 0, 64) / M;
+  q1 = __shfl(c0y, 0, 64) / M;
+  q2 = __shfl(c0z, 0, 64) / M;
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Kicks.  INIT: positions XA as they are, velocities VA, one kick (the opening
 // kick of a call's first step).  Otherwise positions XB + (h/2) Q2 (jump 2)
 // are stored to XA, velocities VB, and kicks = 1 (closing kick of a call's
 // last step) or 2 (closing kick, then the next step's opening kick).
+//
+// check: the next step's pair check (close_encounter_p, src/sim.cuh:589-613)
+// rides on the kick's separations, which are those of the next step's start
+// positions XA: every pair j < k inside its changeover radius (r^2 <
+// max(r_c,j, r_c,k)^2) is appended to the pair list (count ctl[par]) and both
+// bodies flagged; pairs closer than the sum of their physical radii are
+// counted.  The kick loop only marks the slots with r^2 below the system's
+// bound lim (its largest r_c^2, and the overlap bound with its largest
+// radius; mbmax = [S][2]: largest r_c^2, largest radius), which every pair
+// that passes either exact test is below (rounding is monotonic); the marked
+// slots are then tested exactly.
 template <bool INIT>
 __global__ void __launch_bounds__(MB_BLOCK)
-    mb_close_kernel(MBShape sh, const double *__restrict__ bp_all, double *__restrict__ xa_all,
-                    double *__restrict__ va_all, const double *__restrict__ xb_all,
-                    const double *__restrict__ vb_all, int kicks, int last,
-                    unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
-                    StepConst sc, double M) {
+    mb_close_kernel(MBShape sh, const double *__restrict__ bp_all, const double *__restrict__ mbmax_all,
+                    double *__restrict__ xa_all, double *__restrict__ va_all, const double *__restrict__ xb_all,
+                    const double *__restrict__ vb_all, int kicks, int last, int check, int par,
+                    int *__restrict__ flag_all, int *__restrict__ edge_all, unsigned int *__restrict__ ctl_all,
+                    unsigned int *__restrict__ status, unsigned long long *__restrict__ stats, StepConst sc,
+                    double M) {
   extern __shared__ double lds[];
   const int N = sh.N, s = blockIdx.y;
-  // the pair list of this step has been consumed by the drift: empty it
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_all[4 * s] = 0u;
   const BP bp = bp_of(bp_all, s, N);
+  int *flag = flag_all + (size_t)s * N;
+  int *edge = edge_all + (size_t)s * MB_EMAX * 2;
+  unsigned int *npairs = ctl_all + 4 * s + par;
   double *XA = xa_all + (size_t)s * N * 3;
   double *VA = va_all + (size_t)s * N * 3;
   const double *XB = xb_all + (size_t)s * N * 3;
@@ -279,14 +370,24 @@ __global__ void __launch_bounds__(MB_BLOCK)
   const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
   const int i = (int)blockIdx.x * sh.TI + ti;
   const bool act = i < N;
+  unsigned int ov = 0;  // overlapping pairs found by this lane
+  unsigned int mark0 = 0u, mark1 = 0u;  // marked slots of this lane's chunk in windows 0 and 1
   // this lane's body and the first window, loaded ahead of the momentum sum
   double xi = 0.0, yi = 0.0, zi = 0.0, loi = 0.0, wvi = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+  double rc2i = 0.0, radi = 0.0, lim = -1.0;  // lim < 0: no slot is marked
   if (act) {
     xi = X[3 * i + 0];
     yi = X[3 * i + 1];
     zi = X[3 * i + 2];
     loi = bp.lo[i];
     wvi = bp.winv[i];
+    if (check) {
+      rc2i = bp.rc2[i];
+      radi = bp.rad[i];
+      const double so = radi + mbmax_all[2 * s + 1];
+      const double ol = so * so, rm = mbmax_all[2 * s];
+      lim = rm > ol ? rm : ol;
+    }
     if (cs == 0) {
       vx = V[3 * i + 0];
       vy = V[3 * i + 1];
@@ -357,6 +458,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
     if (c >= sh.nch) continue;
     double px = 0.0, py = 0.0, pz = 0.0;
     if (act) {
+      unsigned int mark = 0u;  // slots of this chunk below lim
 #pragma unroll
       for (int hg = 0; hg < MB_CHUNK / MB_DIVG; ++hg) {
         double dx[MB_DIVG], dy[MB_DIVG], dz[MB_DIVG], num[MB_DIVG], den[MB_DIVG], w[MB_DIVG];
@@ -370,9 +472,10 @@ __global__ void __launch_bounds__(MB_BLOCK)
           const bool live = k < N && k != i;
           const double ex = sx[k3 + 0] - xi, ey = sx[k3 + 1] - yi, ez = sx[k3 + 2] - zi;
           const double lok = slo[k1], wvk = swv[k1];
+          const double r2 = dot3(ex, ey, ez, ex, ey, ez);
+          mark |= (r2 < lim ? 1u : 0u) << (hg * MB_DIVG + q);
           double nq, dq;
-          kick_nd_sel(dot3(ex, ey, ez, ex, ey, ez), sc.eps2, sm[k1], loi > lok ? loi : lok,
-                      wvi < wvk ? wvi : wvk, nq, dq);
+          kick_nd_sel(r2, sc.eps2, sm[k1], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, nq, dq);
           dx[q] = live ? ex : 0.0;
           dy[q] = live ? ey : 0.0;
           dz[q] = live ? ez : 0.0;
@@ -387,6 +490,10 @@ __global__ void __launch_bounds__(MB_BLOCK)
           pz = fma(w[q], dz[q], pz);
         }
       }
+      if (w0 == 0)
+        mark0 = mark;
+      else
+        mark1 = mark;
     }
     double *pp = part + ((size_t)c * sh.TI + ti) * 3;
     pp[0] = px;
@@ -443,6 +550,41 @@ __global__ void __launch_bounds__(MB_BLOCK)
                   isfinite(vz)))
       atomicOr(status + s, ST_NAN);
   }
+  // the marked slots' exact tests, pairs i < k by the lower index's lane,
+  // after the kicks (so that nothing of the kick loop is held for them): the
+  // partner's position formed as it was staged, the separation as the kick
+  // formed it (partner minus own body)
+  for (int wi = 0; wi < 2; ++wi) {
+    unsigned int mark = wi == 0 ? mark0 : mark1;
+    const int c = wi * sh.CS + cs;
+    while (mark != 0u) {
+      const int k = c * MB_CHUNK + __ffs(mark) - 1;
+      mark &= mark - 1u;
+      if (k <= i || k >= N) continue;
+      double xk = X[3 * k + 0], yk = X[3 * k + 1], zk = X[3 * k + 2];
+      if (!INIT) {
+        xk = fma(sc.hh, q0, xk);
+        yk = fma(sc.hh, q1, yk);
+        zk = fma(sc.hh, q2, zk);
+      }
+      const double ex = xk - xi, ey = yk - yi, ez = zk - zi;
+      const double r2 = dot3(ex, ey, ez, ex, ey, ez);
+      const double sr = radi + bp.rad[k];
+      if (r2 < sr * sr) ++ov;
+      const double rc2k = bp.rc2[k];
+      if (r2 < (rc2i > rc2k ? rc2i : rc2k)) {
+        const unsigned int slot = atomicAdd(npairs, 1u);
+        if (slot < (unsigned int)MB_EMAX) {
+          edge[2 * slot] = i;
+          edge[2 * slot + 1] = k;
+        }
+        flag[i] = 1;
+        flag[k] = 1;
+      }
+    }
+  }
+  if (ov) atomicOr(status + s, ST_OVERLAP);
+  if (stats) wave_add(stats + S_OVERLAP, ov);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,126 +729,6 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
   big = __any(mybig);
   __syncthreads();
   return G;
-}
-
-// Momentum after the opening kick, jump 1 (stored to Y by the body's owner)
-// and the exact pair check (close_encounter_p, src/sim.cuh:589-613) of every
-// pair j < k against r_c^2 of the pair; pairs inside are appended to the pair
-// list and both bodies flagged.  Also counts pairs closer than the sum of
-// their physical radii.
-__global__ void __launch_bounds__(MB_BLOCK)
-    mb_open_kernel(MBShape sh, const double *__restrict__ bp_all, const double *__restrict__ xa_all,
-                   const double *__restrict__ va_all, double *__restrict__ y_all,
-                   int *__restrict__ flag_all, int *__restrict__ edge_all,
-                   unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
-                   unsigned long long *__restrict__ stats, StepConst sc, double M) {
-  extern __shared__ double lds[];
-  const int N = sh.N, s = blockIdx.y;
-  const BP bp = bp_of(bp_all, s, N);
-  const double *XA = xa_all + (size_t)s * N * 3;
-  const double *VA = va_all + (size_t)s * N * 3;
-  double *Y = y_all + (size_t)s * N * 3;
-  int *flag = flag_all + (size_t)s * N;
-  int *edge = edge_all + (size_t)s * MB_EMAX * 2;
-  unsigned int *ctl = ctl_all + 4 * s;
-  const int WB = sh.CS * MB_CHUNK;
-  double *Qs = lds;
-  double *sx = Qs + 3;                 // partner positions after jump 1, lds3 layout
-  double *src2 = sx + 3 * WB + sh.CS;  // per-body scalars, lds1 layout
-  double *srad = src2 + WB + sh.CS;
-  const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
-  const int i = (int)blockIdx.x * sh.TI + ti;
-  const bool act = i < N;
-  unsigned int ov = 0;
-  // this lane's body and the first window, loaded ahead of the momentum sum
-  double xi = 0.0, yi = 0.0, zi = 0.0, rc2i = 0.0, radi = 0.0;
-  if (act) {
-    xi = XA[3 * i + 0];
-    yi = XA[3 * i + 1];
-    zi = XA[3 * i + 2];
-    rc2i = bp.rc2[i];
-    radi = bp.rad[i];
-  }
-  double pf[MB_PF][5], pm[MB_PF], pv[MB_PF][3];
-#pragma unroll
-  for (int u = 0; u < MB_PF; ++u) {
-    const int k = (int)threadIdx.x + u * MB_BLOCK;
-    if (k < WB && k < N) {
-      pf[u][0] = XA[3 * k + 0];
-      pf[u][1] = XA[3 * k + 1];
-      pf[u][2] = XA[3 * k + 2];
-      pf[u][3] = bp.rc2[k];
-      pf[u][4] = bp.rad[k];
-      pm[u] = bp.m[k];
-      pv[u][0] = VA[3 * k + 0];
-      pv[u][1] = VA[3 * k + 1];
-      pv[u][2] = VA[3 * k + 2];
-    }
-  }
-  // the velocities pass through the partner window's LDS (sx, src2), which
-  // the window loop below fills after the barrier wg_momentum ends with
-  wg_momentum(VA, bp.m, sh, M, pm, pv, sx, src2, Qs);
-  const double q0 = Qs[0], q1 = Qs[1], q2 = Qs[2];
-  if (act) {
-    xi = fma(sc.hh, q0, xi);
-    yi = fma(sc.hh, q1, yi);
-    zi = fma(sc.hh, q2, zi);
-    if (cs == 0) {
-      Y[3 * i + 0] = xi;
-      Y[3 * i + 1] = yi;
-      Y[3 * i + 2] = zi;
-    }
-  }
-  auto stage = [&](int k, double xk, double yk, double zk, double rc2k, double radk) {
-    sx[lds3(k) + 0] = fma(sc.hh, q0, xk);
-    sx[lds3(k) + 1] = fma(sc.hh, q1, yk);
-    sx[lds3(k) + 2] = fma(sc.hh, q2, zk);
-    src2[lds1(k)] = rc2k;
-    srad[lds1(k)] = radk;
-  };
-  for (int w0 = 0; w0 < N; w0 += WB) {
-    __syncthreads();
-    const int nw = min(WB, N - w0);
-    if (w0 == 0) {
-#pragma unroll
-      for (int u = 0; u < MB_PF; ++u) {
-        const int k = (int)threadIdx.x + u * MB_BLOCK;
-        if (k < nw) stage(k, pf[u][0], pf[u][1], pf[u][2], pf[u][3], pf[u][4]);
-      }
-    } else {
-      for (int k = threadIdx.x; k < nw; k += blockDim.x) {
-        const int kk = w0 + k;
-        stage(k, XA[3 * kk + 0], XA[3 * kk + 1], XA[3 * kk + 2], bp.rc2[kk], bp.rad[kk]);
-      }
-    }
-    __syncthreads();
-    const int c = w0 / MB_CHUNK + cs;
-    if (!act || c >= sh.nch || (c + 1) * MB_CHUNK <= i + 1) continue;
-#pragma unroll
-    for (int q = 0; q < MB_CHUNK; ++q) {
-      const int k = c * MB_CHUNK + q;
-      if (k <= i || k >= N) continue;
-      const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
-      const double dx = sx[k3 + 0] - xi;
-      const double dy = sx[k3 + 1] - yi;
-      const double dz = sx[k3 + 2] - zi;
-      const double r2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double sr = radi + srad[k1];
-      if (r2 < sr * sr) ++ov;
-      const double rc2k = src2[k1];
-      if (r2 < (rc2i > rc2k ? rc2i : rc2k)) {
-        const unsigned int slot = atomicAdd(ctl + 0, 1u);
-        if (slot < (unsigned int)MB_EMAX) {
-          edge[2 * slot] = i;
-          edge[2 * slot + 1] = k;
-        }
-        flag[i] = 1;
-        flag[k] = 1;
-      }
-    }
-  }
-  if (ov) atomicOr(status + s, ST_OVERLAP);
-  if (stats) wave_add(stats + S_OVERLAP, ov);
 }
 
 // ---------------------------------------------------------------------------
@@ -908,33 +930,50 @@ __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, do
   return st;
 }
 
-// Drift: workgroups [0, nK) drift the bodies in no pair on their Kepler
-// orbits (and clear the flags for the next step); each of the others builds
-// the encounter groups from the step's pair list in its LDS and takes every
-// nG-th group (LDS as build_groups lists).
+// Drift: every workgroup (one wavefront) first forms Q1, the momentum after
+// the opening kick (wave_momentum), for jump 1: a body's drift starts from
+// XA + (h/2) Q1.  Workgroups [0, nK) then drift the bodies in no pair on
+// their Kepler orbits (and clear the flags for the next step's check); each
+// of the others builds the encounter groups from the step's pair list (count
+// ctl[par], written by the previous kick) in its LDS and takes every nG-th
+// group (LDS as build_groups lists, or the momentum's staging before it).
+// The other pair count, ctl[par ^ 1], is emptied here for the closing kick,
+// which appends the next step's pairs there.
 __global__ void __launch_bounds__(64)
-    mb_drift_kernel(MBShape sh, int nK, int nG, const double *__restrict__ bp_all,
-                    const double *__restrict__ y_all, const double *__restrict__ va_all,
+    mb_drift_kernel(MBShape sh, int nK, int nG, int par, const double *__restrict__ bp_all,
+                    const double *__restrict__ xa_all, const double *__restrict__ va_all,
                     double *__restrict__ xb_all, double *__restrict__ vb_all,
                     int *__restrict__ flag_all, const int *__restrict__ edge_all,
-                    const unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
-                    unsigned long long *__restrict__ stats, StepConst sc) {
-  extern __shared__ int ilds[];
+                    unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
+                    unsigned long long *__restrict__ stats, StepConst sc, double M) {
+  extern __shared__ double dlds[];
+  int *ilds = (int *)dlds;
   const int N = sh.N, s = blockIdx.y;
-  const double *Y = y_all + (size_t)s * N * 3;
+  const BP bp = bp_of(bp_all, s, N);
+  const double *XA = xa_all + (size_t)s * N * 3;
   const double *VA = va_all + (size_t)s * N * 3;
   double *XB = xb_all + (size_t)s * N * 3;
   double *VB = vb_all + (size_t)s * N * 3;
+  double *msv = dlds, *msm = dlds + 3 * sh.CS * MB_CHUNK + sh.CS;  // momentum staging
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_all[4 * s + (par ^ 1)] = 0u;
   if ((int)blockIdx.x < nK) {
     int *flag = flag_all + (size_t)s * N;
     const int i = (int)blockIdx.x * 64 + (int)threadIdx.x;
+    const int ic = i < N ? i : N - 1;
+    // the body's own loads, in flight with the momentum sum's
+    const int fl = flag[ic];
+    double x = XA[3 * ic + 0], y = XA[3 * ic + 1], z = XA[3 * ic + 2];
+    double vx = VA[3 * ic + 0], vy = VA[3 * ic + 1], vz = VA[3 * ic + 2];
+    double q0, q1, q2;
+    wave_momentum(VA, bp.m, sh, M, msv, msm, q0, q1, q2);
     if (i >= N) return;
-    if (flag[i]) {
+    if (fl) {
       flag[i] = 0;
       return;
     }
-    double x = Y[3 * i + 0], y = Y[3 * i + 1], z = Y[3 * i + 2];
-    double vx = VA[3 * i + 0], vy = VA[3 * i + 1], vz = VA[3 * i + 2];
+    x = fma(sc.hh, q0, x);
+    y = fma(sc.hh, q1, y);
+    z = fma(sc.hh, q2, z);
     bool ok;
     kepler_drift_series(x, y, z, vx, vy, vz, sc.mu, sc.h, ok);
     if (!ok) atomicOr(status + s, ST_NEWTON);
@@ -950,13 +989,15 @@ __global__ void __launch_bounds__(64)
   // the pair count and the first 64 pairs (2 MB_EMAX entries are there to
   // read; those past the count are not used) in flight together
   const int e0 = edge[threadIdx.x], e1 = edge[threadIdx.x + 64];
-  const int ne = (int)ctl_all[4 * s];
+  const int ne = (int)ctl_all[4 * s + par];
   const bool lead = (int)blockIdx.x == nK && threadIdx.x == 0;
   if (ne == 0) return;
   if (ne > MB_EMAX) {
     if (lead) atomicOr(status + s, ST_LIMIT);
     return;
   }
+  double q[3];
+  wave_momentum(VA, bp.m, sh, M, msv, msm, q[0], q[1], q[2]);
   unsigned int *bm = (unsigned int *)ilds;
   int *pre = ilds + 64, *fl = pre + 64, *L = fl + 2 * MB_EMAX, *ce = L + 2 * MB_EMAX;
   int *cnt = ce + 2 * MB_EMAX, *gst = cnt + 2 * MB_EMAX, *gm = gst + 2 * MB_EMAX + 1;
@@ -969,7 +1010,6 @@ __global__ void __launch_bounds__(64)
       atomicAdd(stats + S_MGBODIES, (unsigned long long)gst[G]);
     }
   }
-  const BP bp = bp_of(bp_all, s, N);
   const int lane = (int)threadIdx.x;
   unsigned int nrhs = 0;
   uint32_t st = 0;
@@ -986,7 +1026,7 @@ __global__ void __launch_bounds__(64)
     double mj = 0.0, loj = 0.0, wvj = 0.0;
     if (j >= 0) {
       for (int c = 0; c < 3; ++c) {
-        y[c] = Y[3 * j + c];
+        y[c] = fma(sc.hh, q[c], XA[3 * j + c]);
         y[3 + c] = VA[3 * j + c];
       }
       mj = bp.m[j];
@@ -1016,60 +1056,55 @@ size_t close_lds(const MBShape &sh) {
   return ((size_t)sh.nch * sh.TI * 3 + 3 + (3 * WB + sh.CS) + 3 * (WB + sh.CS)) * sizeof(double);
 }
 
-size_t open_lds(const MBShape &sh) {
+// The drift's LDS: the group building's arrays (ints) or, before them, the
+// momentum staging (3 WB + CS and WB + CS doubles), whichever is larger.
+size_t drift_lds(const MBShape &sh) {
+  const size_t groups = ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int);
   const size_t WB = (size_t)sh.CS * MB_CHUNK;
-  return ((size_t)3 + (3 * WB + sh.CS) + 2 * (WB + sh.CS)) * sizeof(double);
+  const size_t mom = (4 * WB + 2 * (size_t)sh.CS) * sizeof(double);
+  return std::max(groups, mom);
 }
 
-size_t drift_lds(const MBShape &) { return ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int); }
-
-int launch_close(Ctx *c, bool init, int kicks, int last) {
+// The kicks of one launch; check: with the pair check of the next step, whose
+// pairs are counted in parity par's counter.
+int launch_close(Ctx *c, bool init, int kicks, int last, int check, int par) {
   const MBShape sh = mb_shape(c->NM);
   const dim3 grid((unsigned int)((sh.N + sh.TI - 1) / sh.TI), (unsigned int)c->S);
   const StepConst sc = c->step_const();
+  unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
   if (init)
     mb_close_kernel<true><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
-        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_mctl, c->d_status, sc,
-        c->cfg.central_mass);
+        sh, c->d_bp, c->d_mbmax, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, check, par, c->d_mflag,
+        c->d_medge, c->d_mctl, c->d_status, stats, sc, c->cfg.central_mass);
   else
     mb_close_kernel<false><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
-        sh, c->d_bp, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, c->d_mctl, c->d_status, sc,
-        c->cfg.central_mass);
+        sh, c->d_bp, c->d_mbmax, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, check, par, c->d_mflag,
+        c->d_medge, c->d_mctl, c->d_status, stats, sc, c->cfg.central_mass);
   return c->hip(hipGetLastError(), "many-body kick launch");
 }
 
-int launch_open(Ctx *c) {
-  const MBShape sh = mb_shape(c->NM);
-  const dim3 grid((unsigned int)((sh.N + sh.TI - 1) / sh.TI), (unsigned int)c->S);
-  unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  mb_open_kernel<<<grid, MB_BLOCK, open_lds(sh), c->stream>>>(
-      sh, c->d_bp, c->d_mx, c->d_mv, c->d_my, c->d_mflag, c->d_medge, c->d_mctl, c->d_status, stats,
-      c->step_const(), c->cfg.central_mass);
-  return c->hip(hipGetLastError(), "many-body encounter launch");
-}
-
-int launch_drift(Ctx *c) {
+int launch_drift(Ctx *c, int par) {
   const MBShape sh = mb_shape(c->NM);
   const int nK = (sh.N + 63) / 64;
   const dim3 grid((unsigned int)(nK + MB_DRIFT_GROUP_BLOCKS), (unsigned int)c->S);
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  mb_drift_kernel<<<grid, 64, drift_lds(sh), c->stream>>>(sh, nK, MB_DRIFT_GROUP_BLOCKS, c->d_bp, c->d_my,
+  mb_drift_kernel<<<grid, 64, drift_lds(sh), c->stream>>>(sh, nK, MB_DRIFT_GROUP_BLOCKS, par, c->d_bp, c->d_mx,
                                                           c->d_mv, c->d_mxb, c->d_mvb, c->d_mflag,
                                                           c->d_medge, c->d_mctl, c->d_status, stats,
-                                                          c->step_const());
+                                                          c->step_const(), c->cfg.central_mass);
   return c->hip(hipGetLastError(), "many-body drift launch");
 }
 
-int enqueue_step(Ctx *c, int kicks, int last, KTimer *kt) {
+// Step k of a call (par = k mod 2: its pairs were counted in ctl[par] by the
+// kick before it): the drift, then the kicks with the next step's pair check
+// unless it is the call's last step.
+int enqueue_step(Ctx *c, int kicks, int last, int par, KTimer *kt) {
   int rc;
   if (kt) kt->start();
-  if ((rc = launch_open(c))) return rc;
-  if (kt) kt->stop(c->timing.ms_mb_open, c->timing.n_mb_open);
-  if (kt) kt->start();
-  if ((rc = launch_drift(c))) return rc;
+  if ((rc = launch_drift(c, par))) return rc;
   if (kt) kt->stop(c->timing.ms_mb_drift, c->timing.n_mb_drift);
   if (kt) kt->start();
-  if ((rc = launch_close(c, false, kicks, last))) return rc;
+  if ((rc = launch_close(c, false, kicks, last, last ? 0 : 1, par ^ 1))) return rc;
   if (kt) kt->stop(c->timing.ms_mb_close, c->timing.n_mb_close);
   return MINT_OK;
 }
@@ -1087,7 +1122,8 @@ int mb_graph(Ctx *c, hipGraphExec_t *out) {
   }
   int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
-  for (int k = 0; k < MB_GRAPH_STEPS && !rc; ++k) rc = enqueue_step(c, 2, 0, nullptr);
+  // replays start at even steps of a call (MB_GRAPH_STEPS is even)
+  for (int k = 0; k < MB_GRAPH_STEPS && !rc; ++k) rc = enqueue_step(c, 2, 0, k & 1, nullptr);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
@@ -1113,7 +1149,7 @@ int mb_graph(Ctx *c, hipGraphExec_t *out) {
 
 int upload_params(Ctx *c) {
   const int S = c->S, N = c->NM;
-  std::vector<double> bp((size_t)S * 5 * N);
+  std::vector<double> bp((size_t)S * 5 * N), mx((size_t)S * 2, 0.0);
   for (int s = 0; s < S; ++s) {
     double *b = bp.data() + (size_t)s * 5 * N;
     for (int j = 0; j < N; ++j) {
@@ -1123,10 +1159,16 @@ int upload_params(Ctx *c) {
       b[2 * N + j] = 1.0 / (0.9 * rc);
       b[3 * N + j] = rc * rc;
       b[4 * N + j] = c->rad.empty() ? 0.0 : c->rad[(size_t)s * N + j];
+      // the pair check's bounds: the largest r_c^2 and radius of the system
+      mx[2 * s] = std::max(mx[2 * s], b[3 * N + j]);
+      mx[2 * s + 1] = std::max(mx[2 * s + 1], b[4 * N + j]);
     }
   }
-  return c->hip(hipMemcpy(c->d_bp, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice),
-                "H2D body parameters");
+  int rc = c->hip(hipMemcpy(c->d_bp, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice),
+                  "H2D body parameters");
+  if (rc) return rc;
+  return c->hip(hipMemcpy(c->d_mbmax, mx.data(), mx.size() * sizeof(double), hipMemcpyHostToDevice),
+                "H2D pair-check bounds");
 }
 
 }  // namespace
@@ -1138,7 +1180,7 @@ int mb_alloc(Ctx *c) {
   if ((rc = c->dalloc((void **)&c->d_bp, (size_t)S * 5 * N * sizeof(double), "body parameters"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mxb, b3, "post-drift positions"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mvb, b3, "post-drift velocities"))) return rc;
-  if ((rc = c->dalloc((void **)&c->d_my, b3, "drift-start positions"))) return rc;
+  if ((rc = c->dalloc((void **)&c->d_mbmax, (size_t)S * 2 * sizeof(double), "pair-check bounds"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mflag, (size_t)S * N * sizeof(int), "encounter flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_medge, (size_t)S * MB_EMAX * 2 * sizeof(int), "encounter pairs"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mctl, (size_t)S * 4 * sizeof(unsigned int), "group counters"))) return rc;
@@ -1159,19 +1201,23 @@ int run_many(Ctx *c, int64_t nsteps) {
   int rc;
   KTimer kt(c);
   KTimer *ktp = timing_on(c) ? &kt : nullptr;
+  // both pair counts empty (the first kick counts step 0's pairs in ctl[0])
+  if ((rc = c->hip(hipMemsetAsync(c->d_mctl, 0, (size_t)c->S * 4 * sizeof(unsigned int), c->stream),
+                   "pair counts clear")))
+    return rc;
   if (ktp) ktp->start();
-  if ((rc = launch_close(c, true, 1, 0))) return rc;
+  if ((rc = launch_close(c, true, 1, 0, 1, 0))) return rc;
   if (ktp) ktp->stop(c->timing.ms_mb_close, c->timing.n_mb_close);
-  int64_t rest = nsteps - 1;
+  int64_t rest = nsteps - 1, k = 0;
   if (c->cfg.use_graph && !ktp && rest >= MB_GRAPH_STEPS) {
     hipGraphExec_t ex = nullptr;
     if ((rc = mb_graph(c, &ex))) return rc;
-    for (; rest >= MB_GRAPH_STEPS; rest -= MB_GRAPH_STEPS)
+    for (; rest >= MB_GRAPH_STEPS; rest -= MB_GRAPH_STEPS, k += MB_GRAPH_STEPS)
       if ((rc = c->hip(hipGraphLaunch(ex, c->stream), "graph launch"))) return rc;
   }
-  for (; rest > 0; --rest)
-    if ((rc = enqueue_step(c, 2, 0, ktp))) return rc;
-  if ((rc = enqueue_step(c, 1, 1, ktp))) return rc;
+  for (; rest > 0; --rest, ++k)
+    if ((rc = enqueue_step(c, 2, 0, (int)(k & 1), ktp))) return rc;
+  if ((rc = enqueue_step(c, 1, 1, (int)(k & 1), ktp))) return rc;
   if ((rc = c->hip(hipStreamSynchronize(c->stream), "many-body step"))) return rc;
   std::vector<unsigned int> st((size_t)c->S);
   if ((rc = c->hip(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(unsigned int), hipMemcpyDeviceToHost),

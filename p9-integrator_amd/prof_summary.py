@@ -8,6 +8,8 @@
         dispatch of the kernels matching the substring
     python prof_summary.py csvcounter <counter_collection.csv> [kernel-substring]
         every dispatch's counter values (CSV), for the kernels matching
+    python prof_summary.py csvcountermean <counter_collection.csv> [kernel-substring]
+        per-kernel mean of each counter per dispatch (CSV)
     python prof_summary.py csvtimeline <kernel_trace.csv> <kernel-substring> [index] [pad-ms]
         the dispatches around one dispatch of the matching kernels (default
         the last), with start and end relative to it and their queues
@@ -98,9 +100,24 @@ def csvcounter(path, filt=""):
         print(f"dispatch {d} `{k}`: " + ", ".join(f"{c} {v:.6g}" for c, v in sorted(cs.items())))
 
 
+def csvcountermean(path, filt=""):
+    """Mean per dispatch of each counter, per kernel (a markdown table)."""
+    rows = list(csv.DictReader(open(path)))
+    per = defaultdict(lambda: defaultdict(dict))
+    for r in rows:
+        if filt in r["Kernel_Name"]:
+            per[short(r["Kernel_Name"])][r["Counter_Name"]][int(r["Dispatch_Id"])] = float(r["Counter_Value"])
+    print("| kernel | counter | mean per dispatch | dispatches |")
+    print("|---|---|---|---|")
+    for k in sorted(per):
+        for c in sorted(per[k]):
+            v = list(per[k][c].values())
+            print(f"| `{k}` | {c} | {sum(v) / len(v):.6g} | {len(v)} |")
+
+
 if __name__ == "__main__":
     cmd, path = sys.argv[1], sys.argv[2]
     filt = sys.argv[3] if len(sys.argv) > 3 else ""
     {"trace": lambda: trace(path), "counter": lambda: counter(path, filt), "csvtrace": lambda: csvtrace(path, filt),
-     "csvcounter": lambda: csvcounter(path, filt),
+     "csvcounter": lambda: csvcounter(path, filt), "csvcountermean": lambda: csvcountermean(path, filt),
      "csvtimeline": lambda: csvtimeline(path, filt, *sys.argv[4:])}[cmd]()
