@@ -726,12 +726,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
   double hh = sc.hh;
   asm volatile("" : "+v"(hh));
   for (int k = 0; k < C; ++k) {
-    // The step's jump momenta and check positions are loaded up front
-    // (uniform, so one round of scalar loads whose latency overlaps the kick;
-    // loaded where each value is used, the loads would sit behind the
-    // divergent branches and each wait would be exposed).  The kick positions
-    // are loaded after the Kepler drift: held across it, they overflow the
-    // scalar registers into vector-lane spills (1.3 % slower).
+    // The step's jump momenta are loaded up front (uniform, so one round of
+    // scalar loads whose latency overlaps the kick; loaded where each value
+    // is used, the loads would sit behind the divergent branches and each
+    // wait would be exposed).  The check positions are loaded where the
+    // check runs, which most steps skip (far): held from the top, they take
+    // scalar registers across the step for nothing (1 % slower at 128-step
+    // windows).  The kick positions are loaded after the Kepler drift: held
+    // across it, they overflow the scalar registers into vector-lane spills
+    // (1.3 % slower).
     const double *rg = rsys + (size_t)__builtin_amdgcn_readfirstlane(k) * REC;
     double r[REC];
 #pragma unroll
@@ -748,7 +751,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
         x = fma(hh, r[R_Q1(NM) + 0], x);
         y = fma(hh, r[R_Q1(NM) + 1], y);
         z = fma(hh, r[R_Q1(NM) + 2], z);
-        if (!far[p] && encounter_tp<NM>(x, y, z, r + R_Y(NM), prm)) {
+        bool enc = false;
+        if (!far[p]) enc = encounter_tp<NM>(x, y, z, rg + R_Y(NM), prm);
+        if (enc) {
           // after the queue head (qcount[0], final before this kernel), by
           // the count of mid-window deferrals (qcount[4])
           const unsigned int slot = qcount[0] + atomicAdd(qcount + 4, 1u);
