@@ -732,9 +732,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
     // wait would be exposed).  The check positions are loaded where the
     // check runs, which most steps skip (far): held from the top, they take
     // scalar registers across the step for nothing (1 % slower at 128-step
-    // windows).  The kick positions are loaded after the Kepler drift: held
-    // across it, they overflow the scalar registers into vector-lane spills
-    // (1.3 % slower).
+    // windows).  The second jump momentum and the kick positions are loaded
+    // after the Kepler drift: held across it, the momentum costs 1 % and the
+    // positions overflow the scalar registers into vector-lane spills (1.3 %
+    // slower).
     const double *rg = rsys + (size_t)__builtin_amdgcn_readfirstlane(k) * REC;
     double r[REC];
 #pragma unroll
@@ -766,9 +767,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(P == 1
           double xe[3 * NM];
 #pragma unroll
           for (int i = 0; i < 3 * NM; ++i) xe[i] = rg[R_XE(NM) + i];
-          x = fma(hh, r[R_Q2(NM) + 0], x);
-          y = fma(hh, r[R_Q2(NM) + 1], y);
-          z = fma(hh, r[R_Q2(NM) + 2], z);
+          x = fma(hh, rg[R_Q2(NM) + 0], x);
+          y = fma(hh, rg[R_Q2(NM) + 1], y);
+          z = fma(hh, rg[R_Q2(NM) + 2], z);
           double ax, ay, az;
           accel_tp<NM>(x, y, z, xe, prm, sc.eps2, ax, ay, az, far[p]);
           A[p][0] = ax; A[p][1] = ay; A[p][2] = az;
