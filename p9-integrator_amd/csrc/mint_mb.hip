@@ -59,7 +59,7 @@ struct MBShape {
 
 constexpr int MB_BLOCK = 256;
 constexpr int MB_MAXCS = 64;   // chunk slots: a staged window holds up to 64 x 16 bodies
-constexpr int MB_DRIFT_GROUP_BLOCKS = 32;  // workgroups that walk the group list
+constexpr int MB_DRIFT_GROUP_BLOCKS = 16;  // workgroups that walk the group list (two wavefronts each)
 constexpr int MB_GRAPH_STEPS = 64;
 
 MBShape mb_shape(int N) {
@@ -271,11 +271,11 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
 // 64 lanes; ends with a barrier, after which the caller may reuse the LDS.
 __device__ void wave_momentum(const double *__restrict__ V, const double *__restrict__ m, const MBShape &sh,
                               double M, double *sv, double *sm, double &q0, double &q1, double &q2) {
-  const int t = (int)threadIdx.x, N = sh.N, WB = sh.CS * MB_CHUNK;
+  const int t = (int)threadIdx.x & 63, N = sh.N, WB = sh.CS * MB_CHUNK;
   double c0x = 0.0, c0y = 0.0, c0z = 0.0, c1x = 0.0, c1y = 0.0, c1z = 0.0;
   for (int b0 = 0, pass = 0; b0 < N; b0 += WB, ++pass) {
     const int nw = min(WB, N - b0);
-    if (pass > 0) __syncthreads();  // the previous window has been read
+    if (pass > 0) wave_lds_sync();  // the previous window has been read
     // every load of the window in flight at once (a window holds at most
     // MB_MAXCS x 16 = 1024 bodies, 16 a lane)
     constexpr int U = MB_MAXCS * MB_CHUNK / 64;
@@ -298,7 +298,7 @@ __device__ void wave_momentum(const double *__restrict__ V, const double *__rest
         sm[lds1(k)] = lm[u];
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (t * MB_CHUNK < nw) {
       double px, py, pz;
       chunk_momentum(sv, sm, t, nw, px, py, pz);
@@ -322,7 +322,7 @@ __device__ void wave_momentum(const double *__restrict__ V, const double *__rest
   q0 = __shfl(c0x, 0, 64) / M;
   q1 = __shfl(c0y, 0, 64) / M;
   q2 = __shfl(c0z, 0, 64) / M;
-  __syncthreads();
+  wave_lds_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -620,16 +620,16 @@ static_assert(2 * MB_EMAX >= 128, "the first 64 pairs are read ahead of the pair
 __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0, int e1,
                             unsigned int *bm, int *pre, int *fl, int *L, int *ce, int *cnt, int *gst,
                             int *gm, bool &big) {
-  const int lane = threadIdx.x;  // one wavefront
+  const int lane = (int)threadIdx.x & 63;  // one wavefront
   const int n2 = 2 * ne, nwd = (N + 31) >> 5;
   bm[lane] = 0u;
-  __syncthreads();
+  wave_lds_sync();
   for (int p = lane; p < n2; p += 64) {
     const int b = p < 64 ? e0 : (p < 128 ? e1 : edge[p]);
     ce[p] = b;
     atomicOr(&bm[b >> 5], 1u << (b & 31));
   }
-  __syncthreads();
+  wave_lds_sync();
   const unsigned int word = lane < nwd ? bm[lane] : 0u;
   const int pc = __popc(word);
   const int inc = wave_incl_scan(pc, lane);
@@ -644,7 +644,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
       w &= w - 1u;
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   for (int p = lane; p < n2; p += 64) {
     const int b = ce[p];
     ce[p] = pre[b >> 5] + __popc(bm[b >> 5] & ((1u << (b & 31)) - 1u));
@@ -653,7 +653,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
     L[c] = c;
     cnt[c] = 0;
   }
-  __syncthreads();
+  wave_lds_sync();
   for (;;) {
     int changed = 0;
     for (int e = lane; e < ne; e += 64) {
@@ -667,7 +667,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
         changed = 1;
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (!__any(changed)) break;
   }
   if (F <= 64) {
@@ -687,11 +687,11 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
     }
     if (lane == 0) gst[g] = F;
     big = false;
-    __syncthreads();
+    wave_lds_sync();
     return g;
   }
   for (int c = lane; c < F; c += 64) atomicAdd(&cnt[L[c]], 1);
-  __syncthreads();
+  wave_lds_sync();
   // scan over the compact bodies of (root, size at root), PER a lane
   const int PER = (F + 63) >> 6;
   const int c0 = lane * PER;
@@ -708,7 +708,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
   const int ir = wave_incl_scan(sr, lane), iz = wave_incl_scan(sz, lane);
   const int G = __shfl(ir, 63, 64);
   int xr = ir - sr, xz = iz - sz;
-  __syncthreads();
+  wave_lds_sync();
   for (int u = 0; u < PER; ++u) {
     const int c = c0 + u;
     if (c < F && L[c] == c) {
@@ -718,7 +718,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
       xz += size;
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   for (int c = lane; c < F; c += 64) {
     const int r = L[c];
     int rank = 0;
@@ -727,7 +727,7 @@ __device__ int build_groups(int N, int ne, const int *__restrict__ edge, int e0,
   }
   if (lane == 0) gst[G] = F;
   big = __any(mybig);
-  __syncthreads();
+  wave_lds_sync();
   return G;
 }
 
@@ -930,16 +930,25 @@ __device__ uint32_t bs_group_wave(double *y, int a, int GP, int gs, double h, do
   return st;
 }
 
-// Drift: every workgroup (one wavefront) first forms Q1, the momentum after
-// the opening kick (wave_momentum), for jump 1: a body's drift starts from
-// XA + (h/2) Q1.  Workgroups [0, nK) then drift the bodies in no pair on
-// their Kepler orbits (and clear the flags for the next step's check); each
-// of the others builds the encounter groups from the step's pair list (count
-// ctl[par], written by the previous kick) in its LDS and takes every nG-th
-// group (LDS as build_groups lists, or the momentum's staging before it).
-// The other pair count, ctl[par ^ 1], is emptied here for the closing kick,
-// which appends the next step's pairs there.
-__global__ void __launch_bounds__(64)
+// Drift, in workgroups of two wavefronts.  Q1, the momentum after the opening
+// kick, gives jump 1: a body's drift starts from XA + (h/2) Q1.  Workgroups
+// [0, nK): each wavefront forms Q1 (wave_momentum, in its own half of the
+// LDS) and drifts its 64 bodies that are in no pair on their Kepler orbits
+// (clearing the flags for the next step's check).  The others: wavefront 0
+// forms Q1 while wavefront 1 builds the encounter groups from the step's
+// pair list (count ctl[par], written by the previous kick), each in its own
+// LDS; then every wavefront takes every 2 nG-th group.  The other pair count,
+// ctl[par ^ 1], is emptied here for the closing kick, which appends the next
+// step's pairs there.  LDS (drift_lds): Q1 (4 doubles), region A (the
+// momentum staging), region B (build_groups' arrays, or the second
+// wavefront's momentum staging in a Kepler workgroup).
+constexpr int MB_DRIFT_BLOCK = 128;
+
+__device__ __forceinline__ size_t mom_doubles(const MBShape &sh) {
+  return 4 * (size_t)sh.CS * MB_CHUNK + 2 * (size_t)sh.CS;
+}
+
+__global__ void __launch_bounds__(MB_DRIFT_BLOCK)
     mb_drift_kernel(MBShape sh, int nK, int nG, int par, const double *__restrict__ bp_all,
                     const double *__restrict__ xa_all, const double *__restrict__ va_all,
                     double *__restrict__ xb_all, double *__restrict__ vb_all,
@@ -947,25 +956,29 @@ __global__ void __launch_bounds__(64)
                     unsigned int *__restrict__ ctl_all, unsigned int *__restrict__ status,
                     unsigned long long *__restrict__ stats, StepConst sc, double M) {
   extern __shared__ double dlds[];
-  int *ilds = (int *)dlds;
   const int N = sh.N, s = blockIdx.y;
+  const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const BP bp = bp_of(bp_all, s, N);
   const double *XA = xa_all + (size_t)s * N * 3;
   const double *VA = va_all + (size_t)s * N * 3;
   double *XB = xb_all + (size_t)s * N * 3;
   double *VB = vb_all + (size_t)s * N * 3;
-  double *msv = dlds, *msm = dlds + 3 * sh.CS * MB_CHUNK + sh.CS;  // momentum staging
+  double *sq = dlds;                           // Q1
+  double *regA = dlds + 4;                     // momentum staging
+  double *regB = regA + mom_doubles(sh);       // group arrays
+  const size_t WBC = 3 * (size_t)sh.CS * MB_CHUNK + sh.CS;  // staged velocities (lds3 layout)
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl_all[4 * s + (par ^ 1)] = 0u;
   if ((int)blockIdx.x < nK) {
     int *flag = flag_all + (size_t)s * N;
-    const int i = (int)blockIdx.x * 64 + (int)threadIdx.x;
+    const int i = (int)blockIdx.x * MB_DRIFT_BLOCK + (int)threadIdx.x;
     const int ic = i < N ? i : N - 1;
     // the body's own loads, in flight with the momentum sum's
     const int fl = flag[ic];
     double x = XA[3 * ic + 0], y = XA[3 * ic + 1], z = XA[3 * ic + 2];
     double vx = VA[3 * ic + 0], vy = VA[3 * ic + 1], vz = VA[3 * ic + 2];
+    double *mr = wv == 0 ? regA : regB;
     double q0, q1, q2;
-    wave_momentum(VA, bp.m, sh, M, msv, msm, q0, q1, q2);
+    wave_momentum(VA, bp.m, sh, M, mr, mr + WBC, q0, q1, q2);
     if (i >= N) return;
     if (fl) {
       flag[i] = 0;
@@ -988,32 +1001,45 @@ __global__ void __launch_bounds__(64)
   const int *edge = edge_all + (size_t)s * MB_EMAX * 2;
   // the pair count and the first 64 pairs (2 MB_EMAX entries are there to
   // read; those past the count are not used) in flight together
-  const int e0 = edge[threadIdx.x], e1 = edge[threadIdx.x + 64];
+  const int e0 = edge[lane], e1 = edge[lane + 64];
   const int ne = (int)ctl_all[4 * s + par];
-  const bool lead = (int)blockIdx.x == nK && threadIdx.x == 0;
+  const bool lead = (int)blockIdx.x == nK && threadIdx.x == 64;  // the first builder's first lane
   if (ne == 0) return;
   if (ne > MB_EMAX) {
     if (lead) atomicOr(status + s, ST_LIMIT);
     return;
   }
-  double q[3];
-  wave_momentum(VA, bp.m, sh, M, msv, msm, q[0], q[1], q[2]);
+  int *ilds = (int *)regB;
   unsigned int *bm = (unsigned int *)ilds;
   int *pre = ilds + 64, *fl = pre + 64, *L = fl + 2 * MB_EMAX, *ce = L + 2 * MB_EMAX;
   int *cnt = ce + 2 * MB_EMAX, *gst = cnt + 2 * MB_EMAX, *gm = gst + 2 * MB_EMAX + 1;
-  bool big = false;
-  const int G = build_groups(N, ne, edge, e0, e1, bm, pre, fl, L, ce, cnt, gst, gm, big);
-  if (lead) {
-    atomicOr(status + s, big ? (ST_MASSIVE_ENC | ST_LIMIT) : ST_MASSIVE_ENC);
-    if (stats) {
-      atomicAdd(stats + S_MGROUPS, (unsigned long long)G);
-      atomicAdd(stats + S_MGBODIES, (unsigned long long)gst[G]);
+  if (wv == 0) {
+    double q0, q1, q2;
+    wave_momentum(VA, bp.m, sh, M, regA, regA + WBC, q0, q1, q2);
+    if (lane == 0) {
+      sq[0] = q0;
+      sq[1] = q1;
+      sq[2] = q2;
+    }
+  } else {
+    bool big = false;
+    const int G = build_groups(N, ne, edge, e0, e1, bm, pre, fl, L, ce, cnt, gst, gm, big);
+    if (lane == 0) ((int *)(sq + 3))[0] = G;
+    if (lead) {
+      atomicOr(status + s, big ? (ST_MASSIVE_ENC | ST_LIMIT) : ST_MASSIVE_ENC);
+      if (stats) {
+        atomicAdd(stats + S_MGROUPS, (unsigned long long)G);
+        atomicAdd(stats + S_MGBODIES, (unsigned long long)gst[G]);
+      }
     }
   }
-  const int lane = (int)threadIdx.x;
+  __syncthreads();
+  const double q[3] = {sq[0], sq[1], sq[2]};
+  const int G = ((const int *)(sq + 3))[0];
   unsigned int nrhs = 0;
   uint32_t st = 0;
-  for (int g = (int)blockIdx.x - nK; g < G; g += nG) {
+  const int nw = 2 * nG;  // group wavefronts
+  for (int g = 2 * ((int)blockIdx.x - nK) + wv; g < G; g += nw) {
     const int g0 = gst[g], gs = gst[g + 1] - g0;
     if (gs > MB_GMAX) continue;
     int GP = 1;  // lanes per row group: the group size rounded up to a power of two
@@ -1056,13 +1082,15 @@ size_t close_lds(const MBShape &sh) {
   return ((size_t)sh.nch * sh.TI * 3 + 3 + (3 * WB + sh.CS) + 3 * (WB + sh.CS)) * sizeof(double);
 }
 
-// The drift's LDS: the group building's arrays (ints) or, before them, the
-// momentum staging (3 WB + CS and WB + CS doubles), whichever is larger.
+// The drift's LDS: Q1 (4 doubles), region A (a momentum staging: 3 WB + CS
+// and WB + CS doubles), region B (the group building's arrays, or a second
+// momentum staging), up to 84 KB at the 2048-body limit (gfx950 gives a
+// workgroup up to 160 KB).
 size_t drift_lds(const MBShape &sh) {
   const size_t groups = ((size_t)64 + 64 + 12 * MB_EMAX + 1) * sizeof(int);
   const size_t WB = (size_t)sh.CS * MB_CHUNK;
   const size_t mom = (4 * WB + 2 * (size_t)sh.CS) * sizeof(double);
-  return std::max(groups, mom);
+  return 4 * sizeof(double) + mom + std::max(groups, mom);
 }
 
 // The kicks of one launch; check: with the pair check of the next step, whose
@@ -1085,10 +1113,12 @@ int launch_close(Ctx *c, bool init, int kicks, int last, int check, int par) {
 
 int launch_drift(Ctx *c, int par) {
   const MBShape sh = mb_shape(c->NM);
-  const int nK = (sh.N + 63) / 64;
-  const dim3 grid((unsigned int)(nK + MB_DRIFT_GROUP_BLOCKS), (unsigned int)c->S);
+  const int nK = (sh.N + MB_DRIFT_BLOCK - 1) / MB_DRIFT_BLOCK;
+  const int nG = MB_DRIFT_GROUP_BLOCKS;
+  const dim3 grid((unsigned int)(nK + nG), (unsigned int)c->S);
+  const size_t lds = drift_lds(sh);  // allowed above 64 KB by mb_alloc
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
-  mb_drift_kernel<<<grid, 64, drift_lds(sh), c->stream>>>(sh, nK, MB_DRIFT_GROUP_BLOCKS, par, c->d_bp, c->d_mx,
+  mb_drift_kernel<<<grid, MB_DRIFT_BLOCK, lds, c->stream>>>(sh, nK, nG, par, c->d_bp, c->d_mx,
                                                           c->d_mv, c->d_mxb, c->d_mvb, c->d_mflag,
                                                           c->d_medge, c->d_mctl, c->d_status, stats,
                                                           c->step_const(), c->cfg.central_mass);
@@ -1184,7 +1214,11 @@ int mb_alloc(Ctx *c) {
   if ((rc = c->dalloc((void **)&c->d_mflag, (size_t)S * N * sizeof(int), "encounter flags"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_medge, (size_t)S * MB_EMAX * 2 * sizeof(int), "encounter pairs"))) return rc;
   if ((rc = c->dalloc((void **)&c->d_mctl, (size_t)S * 4 * sizeof(unsigned int), "group counters"))) return rc;
-  return MINT_OK;
+  // the drift's workgroups take more than 64 KB of LDS (allowed here for the
+  // largest system any context may hold, so contexts of every size launch)
+  return c->hip(hipFuncSetAttribute((const void *)mb_drift_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)drift_lds(mb_shape(MB_MAXN))),
+                "many-body drift LDS size");
 }
 
 int mb_reset(Ctx *c) {
