@@ -258,10 +258,11 @@ int mint_cartesian_to_elements(int64_t n, double mu, const double *x, const doub
                                double *ecc, double *sma);
 
 /* Autotuner (batched form): time the candidate launch shapes on the uploaded
- * batch (each once untimed, then the best of two timed 256-step probes), keep
- * the fastest, then the same for block and TPs per lane on calls of one
- * launch window (a probe of up to 32 steps; only where neither is set in the
- * configuration), and store both in cache_path (NULL = no file).  A shape
+ * batch (each once untimed, then three timed 256-step probes), keep the one
+ * with the fastest median, then the same for block and TPs per lane on calls
+ * of one launch window (probes of up to 32 steps, ranked by the slowest of
+ * three; only where neither is set in the configuration), and store both in
+ * cache_path (NULL = no file).  A shape
  * parameter set in the configuration (block_size, tps_per_lane, chunk_steps
  * > 0) is held at that value.  The state on the device is left unchanged.
  * best[3] = {block, tps/lane, chunk} of calls of more than one window. */
