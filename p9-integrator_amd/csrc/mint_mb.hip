@@ -165,7 +165,10 @@ __device__ __forceinline__ int lds1(int k) { return k + (k >> 4); }      // scal
 // The first window's data is loaded into registers before the momentum sum,
 // so that both sets of loads are in flight together; the jump is applied
 // when the values go to LDS, once the momentum is known.
-constexpr int MB_PF = MB_MAXCS * MB_CHUNK / MB_BLOCK;
+// Threads of a kick workgroup: two per (body, chunk) slot, one per division
+// group of the chunk's 16 partners.
+constexpr int MB_NT = 2 * MB_BLOCK;
+constexpr int MB_PF = MB_MAXCS * MB_CHUNK / MB_NT;
 
 // Chunk t's momentum sum over a staged window of nw bodies (sv in the lds3
 // layout, sm in the lds1 layout): from zero, in index order.  Slots past the
@@ -200,7 +203,7 @@ __device__ __forceinline__ void chunk_momentum(const double *sv, const double *s
 // Q = (sum_j m_j v_j)/M by the whole workgroup.  Masses and velocities pass
 // through LDS a window of WB bodies at a time (sv in the lds3 layout, sm in
 // the lds1 layout; the first window comes from the caller's registers pm, pv,
-// loaded by body index k = threadIdx.x + u MB_BLOCK, so every load is
+// loaded by body index k = threadIdx.x + u MB_NT, so every load is
 // coalesced); lane c of wavefront 0 forms the window's chunk c sum (from zero,
 // index order), and the chunk sums are folded (fold_lanes).  At most two
 // windows: N <= 2048 = 2 WB.  Qs[3] receives Q; the caller's next write to sv
@@ -216,7 +219,7 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
     if (pass == 0) {
 #pragma unroll
       for (int u = 0; u < MB_PF; ++u) {
-        const int k = t + u * MB_BLOCK;
+        const int k = t + u * MB_NT;
         if (k < nw) {
           sv[lds3(k) + 0] = pv[u][0];
           sv[lds3(k) + 1] = pv[u][1];
@@ -225,7 +228,7 @@ __device__ void wg_momentum(const double *__restrict__ V, const double *__restri
         }
       }
     } else {
-      for (int k = t; k < nw; k += MB_BLOCK) {
+      for (int k = t; k < nw; k += MB_NT) {
         const int kk = b0 + k;
         sv[lds3(k) + 0] = V[3 * kk + 0];
         sv[lds3(k) + 1] = V[3 * kk + 1];
@@ -342,7 +345,7 @@ __device__ void wave_momentum(const double *__restrict__ V, const double *__rest
 // that passes either exact test is below (rounding is monotonic); the marked
 // slots are then tested exactly.
 template <bool INIT>
-__global__ void __launch_bounds__(MB_BLOCK)
+__global__ void __launch_bounds__(MB_NT)
     mb_close_kernel(MBShape sh, const double *__restrict__ bp_all, const double *__restrict__ mbmax_all,
                     double *__restrict__ xa_all, double *__restrict__ va_all, const double *__restrict__ xb_all,
                     const double *__restrict__ vb_all, int kicks, int last, int check, int par,
@@ -367,7 +370,10 @@ __global__ void __launch_bounds__(MB_BLOCK)
   double *slo = sm + WB + sh.CS, *swv = slo + WB + sh.CS;
   const double *X = INIT ? XA : XB;
   const double *V = INIT ? VA : VB;
-  const int ti = (int)threadIdx.x % sh.TI, cs = (int)threadIdx.x / sh.TI;
+  // thread = (chunk slot cs, half hf, body ti): the two halves of a slot
+  // take the chunk's two division groups of eight partners
+  const int ti = (int)threadIdx.x % sh.TI, rs = (int)threadIdx.x / sh.TI;
+  const int cs = rs >> 1, hf = rs & 1;
   const int i = (int)blockIdx.x * sh.TI + ti;
   const bool act = i < N;
   unsigned int ov = 0;  // overlapping pairs found by this lane
@@ -388,7 +394,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
       const double ol = so * so, rm = mbmax_all[2 * s];
       lim = rm > ol ? rm : ol;
     }
-    if (cs == 0) {
+    if (rs == 0) {
       vx = V[3 * i + 0];
       vy = V[3 * i + 1];
       vz = V[3 * i + 2];
@@ -397,7 +403,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
   double pf[MB_PF][6], pm[MB_PF], pv[MB_PF][3];
 #pragma unroll
   for (int u = 0; u < MB_PF; ++u) {
-    const int k = (int)threadIdx.x + u * MB_BLOCK;
+    const int k = (int)threadIdx.x + u * MB_NT;
     if (k < WB && k < N) {
       pf[u][0] = X[3 * k + 0];
       pf[u][1] = X[3 * k + 1];
@@ -444,7 +450,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
     if (w0 == 0) {
 #pragma unroll
       for (int u = 0; u < MB_PF; ++u) {
-        const int k = (int)threadIdx.x + u * MB_BLOCK;
+        const int k = (int)threadIdx.x + u * MB_NT;
         if (k < nw) stage(k, pf[u][0], pf[u][1], pf[u][2], pf[u][3], pf[u][4], pf[u][5]);
       }
     } else {
@@ -454,54 +460,82 @@ __global__ void __launch_bounds__(MB_BLOCK)
       }
     }
     __syncthreads();
+    // this thread's chunk and division group: partners c 16 + hf 8 + q; every
+    // thread reaches the barrier below, whether or not its slot is in use
     const int c = w0 / MB_CHUNK + cs;
-    if (c >= sh.nch) continue;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (act) {
-      unsigned int mark = 0u;  // slots of this chunk below lim
+    const bool slot = c < sh.nch;
+    double dx[MB_DIVG], dy[MB_DIVG], dz[MB_DIVG], w[MB_DIVG];
+    if (slot && act) {
+      double num[MB_DIVG], den[MB_DIVG];
+      unsigned int mark = 0u;  // slots of this group below lim
 #pragma unroll
-      for (int hg = 0; hg < MB_CHUNK / MB_DIVG; ++hg) {
-        double dx[MB_DIVG], dy[MB_DIVG], dz[MB_DIVG], num[MB_DIVG], den[MB_DIVG], w[MB_DIVG];
-#pragma unroll
-        for (int q = 0; q < MB_DIVG; ++q) {
-          // every slot is formed and the inert ones (the body itself, slots
-          // past N, whose window entries are not staged) are selected away,
-          // so that the eight pairs' chains interleave without branches
-          const int k = c * MB_CHUNK + hg * MB_DIVG + q;
-          const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
-          const bool live = k < N && k != i;
-          const double ex = sx[k3 + 0] - xi, ey = sx[k3 + 1] - yi, ez = sx[k3 + 2] - zi;
-          const double lok = slo[k1], wvk = swv[k1];
-          const double r2 = dot3(ex, ey, ez, ex, ey, ez);
-          mark |= (r2 < lim ? 1u : 0u) << (hg * MB_DIVG + q);
-          double nq, dq;
-          kick_nd_sel(r2, sc.eps2, sm[k1], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, nq, dq);
-          dx[q] = live ? ex : 0.0;
-          dy[q] = live ? ey : 0.0;
-          dz[q] = live ? ez : 0.0;
-          num[q] = live ? nq : 0.0;
-          den[q] = live ? dq : 1.0;
-        }
-        batched_div<MB_DIVG>(num, den, w);
-#pragma unroll
-        for (int q = 0; q < MB_DIVG; ++q) {
-          px = fma(w[q], dx[q], px);
-          py = fma(w[q], dy[q], py);
-          pz = fma(w[q], dz[q], pz);
-        }
+      for (int q = 0; q < MB_DIVG; ++q) {
+        // every slot is formed and the inert ones (the body itself, slots
+        // past N, whose window entries are not staged) are selected away,
+        // so that the eight pairs' chains interleave without branches
+        const int k = c * MB_CHUNK + hf * MB_DIVG + q;
+        const int kl = k - w0, k3 = lds3(kl), k1 = lds1(kl);
+        const bool live = k < N && k != i;
+        const double ex = sx[k3 + 0] - xi, ey = sx[k3 + 1] - yi, ez = sx[k3 + 2] - zi;
+        const double lok = slo[k1], wvk = swv[k1];
+        const double r2 = dot3(ex, ey, ez, ex, ey, ez);
+        mark |= (r2 < lim ? 1u : 0u) << (hf * MB_DIVG + q);
+        double nq, dq;
+        kick_nd_sel(r2, sc.eps2, sm[k1], loi > lok ? loi : lok, wvi < wvk ? wvi : wvk, nq, dq);
+        dx[q] = live ? ex : 0.0;
+        dy[q] = live ? ey : 0.0;
+        dz[q] = live ? ez : 0.0;
+        num[q] = live ? nq : 0.0;
+        den[q] = live ? dq : 1.0;
       }
+      batched_div<MB_DIVG>(num, den, w);
       if (w0 == 0)
         mark0 = mark;
       else
         mark1 = mark;
+    } else {
+#pragma unroll
+      for (int q = 0; q < MB_DIVG; ++q) {
+        dx[q] = 0.0;
+        dy[q] = 0.0;
+        dz[q] = 0.0;
+        w[q] = 0.0;
+      }
     }
-    double *pp = part + ((size_t)c * sh.TI + ti) * 3;
-    pp[0] = px;
-    pp[1] = py;
-    pp[2] = pz;
+    // the chunk sum from zero in partner order: the first group's terms (half
+    // 0), then, on from its partial through the chunk's slot in part, the
+    // second's (half 1)
+    double *pp = part + ((size_t)(slot ? c : 0) * sh.TI + ti) * 3;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (slot && hf == 0) {
+#pragma unroll
+      for (int q = 0; q < MB_DIVG; ++q) {
+        px = fma(w[q], dx[q], px);
+        py = fma(w[q], dy[q], py);
+        pz = fma(w[q], dz[q], pz);
+      }
+      pp[0] = px;
+      pp[1] = py;
+      pp[2] = pz;
+    }
+    __syncthreads();
+    if (slot && hf == 1) {
+      px = pp[0];
+      py = pp[1];
+      pz = pp[2];
+#pragma unroll
+      for (int q = 0; q < MB_DIVG; ++q) {
+        px = fma(w[q], dx[q], px);
+        py = fma(w[q], dy[q], py);
+        pz = fma(w[q], dz[q], pz);
+      }
+      pp[0] = px;
+      pp[1] = py;
+      pp[2] = pz;
+    }
   }
   __syncthreads();
-  {
+  if ((int)threadIdx.x < MB_BLOCK) {  // the first MB_BLOCK threads: whole wavefronts
     // fold of body b's chunk sums by lanes [b G, (b + 1) G); TI G = MB_BLOCK
     // (G = min(P, 64) = CS); the folds go to acc[TI][3] over the spent window
     const int G = sh.P < 64 ? sh.P : 64;
@@ -528,7 +562,7 @@ __global__ void __launch_bounds__(MB_BLOCK)
     }
   }
   __syncthreads();
-  if (cs == 0 && act) {
+  if (rs == 0 && act) {
     const double ax = sx[3 * ti + 0], ay = sx[3 * ti + 1], az = sx[3 * ti + 2];
     vx = fma(sc.hh, ax, vx);
     vy = fma(sc.hh, ay, vy);
@@ -1101,11 +1135,11 @@ int launch_close(Ctx *c, bool init, int kicks, int last, int check, int par) {
   const StepConst sc = c->step_const();
   unsigned long long *stats = (c->cfg.collect_stats & 1) ? c->d_stats : nullptr;
   if (init)
-    mb_close_kernel<true><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
+    mb_close_kernel<true><<<grid, MB_NT, close_lds(sh), c->stream>>>(
         sh, c->d_bp, c->d_mbmax, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, check, par, c->d_mflag,
         c->d_medge, c->d_mctl, c->d_status, stats, sc, c->cfg.central_mass);
   else
-    mb_close_kernel<false><<<grid, MB_BLOCK, close_lds(sh), c->stream>>>(
+    mb_close_kernel<false><<<grid, MB_NT, close_lds(sh), c->stream>>>(
         sh, c->d_bp, c->d_mbmax, c->d_mx, c->d_mv, c->d_mxb, c->d_mvb, kicks, last, check, par, c->d_mflag,
         c->d_medge, c->d_mctl, c->d_status, stats, sc, c->cfg.central_mass);
   return c->hip(hipGetLastError(), "many-body kick launch");
