@@ -33,10 +33,12 @@
 // (src/sim.cuh:410-551,699-730).  The reference integrates each body against
 // frozen partners; here a group's members are integrated together.
 //
-// Kick sums: lane (i, chunk) accumulates the 16 partners of one chunk from
-// zero, two division groups of 8 with one division each (batched_div), the
-// body itself and bodies past N as inert slots; the chunk sums are then folded
-// by halving strides (fold_chunks).  The momentum sum uses the same chunks and
+// Kick sums: body i's sum over the 16 partners of one chunk is formed from
+// zero in partner order, two division groups of 8 with one division each
+// (batched_div), the body itself and bodies past N as inert slots; two
+// threads share a (body, chunk) slot, one per division group, the second
+// taking the first's partial sum on; the chunk sums are then folded by
+// halving strides (fold_lanes).  The momentum sum uses the same chunks and
 // the same fold.
 #include <hip/hip_runtime.h>
 
