@@ -995,12 +995,12 @@ __device__ __forceinline__ void rhs_group(const double *s, int g, const int *idx
 template <int NM>
 __device__ __noinline__ void gragg_group(const double *y0, const double *f0, double H, int N, int g,
                                          const int *idx, const double *__restrict__ prm,
-                                         double mu, double eps2, double *out) {
+                                         double mu, double eps2, double *out, double *ws) {
   constexpr int D = 6 * NM;
   const int n = 6 * g;
   const double hs = H / (double)N;
   const double h2 = 2.0 * hs;
-  double z0[D], z1[D], f[D];
+  double *z0 = ws, *z1 = ws + D, *f = ws + 2 * D;  // the caller's workspace (3 D doubles)
   for (int i = 0; i < n; ++i) {
     z0[i] = y0[i];
     z1[i] = fma(hs, f0[i], y0[i]);
@@ -1017,20 +1017,30 @@ __device__ __noinline__ void gragg_group(const double *y0, const double *f0, dou
   for (int i = 0; i < n; ++i) out[i] = 0.5 * (z1[i] + fma(hs, f[i], z0[i]));
 }
 
+// Workspace of bs_drift_group, in doubles: f0, cur (D each), the error
+// scales (2 NM), the extrapolation tableau (KMAX D) and gragg_group's (3 D).
+template <int NM>
+constexpr int bs_group_ws() {
+  return (5 + KMAX) * 6 * NM + 2 * NM;
+}
+
 // BS drift of one massive encounter group over h.  Error scale per member:
-// |x_j|, |v_j| at the start of each interval.
+// |x_j|, |v_j| at the start of each interval.  Its arrays live in the
+// caller's workspace ws (bs_group_ws<NM>() doubles, LDS in the ephemeris
+// kernel), so that no private memory is needed.
 template <int NM>
 __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx, double h,
                                                 const double *__restrict__ prm, double mu,
-                                                double eps2, double tol, unsigned int &nrhs) {
+                                                double eps2, double tol, unsigned int &nrhs, double *ws) {
   constexpr int D = 6 * NM;
   const int n = 6 * g;
+  double *f0 = ws, *cur = ws + D, *sc = ws + 2 * D, *R = ws + 2 * D + 2 * NM;
+  double *gws = R + KMAX * D;
   uint32_t st = 0;
   double t = 0.0, H = h;
   int depth = 0;
   while (t < h) {
     if (H > h - t) H = h - t;
-    double f0[D], sc[2 * NM];
     rhs_group<NM>(s, g, idx, prm, mu, eps2, f0);
     ++nrhs;
     for (int a = 0; a < g; ++a) {
@@ -1038,26 +1048,24 @@ __device__ __noinline__ uint32_t bs_drift_group(double *s, int g, const int *idx
       sc[2 * a + 1] =
           sqrt_dist2(dot3(s[6 * a + 3], s[6 * a + 4], s[6 * a + 5], s[6 * a + 3], s[6 * a + 4], s[6 * a + 5]));
     }
-    double R[KMAX][D];
-    double cur[D];
     bool conv = false;
     for (int k = 0; k < KMAX; ++k) {
-      gragg_group<NM>(s, f0, H, 2 << k, g, idx, prm, mu, eps2, cur);
+      gragg_group<NM>(s, f0, H, 2 << k, g, idx, prm, mu, eps2, cur, gws);
       nrhs += 2u << k;
       for (int j = 1; j <= k; ++j) {
         const double fac = bs_fac(j);
         for (int i = 0; i < n; ++i) {
-          const double prev = R[j - 1][i];
-          R[j - 1][i] = cur[i];
+          const double prev = R[(j - 1) * D + i];
+          R[(j - 1) * D + i] = cur[i];
           cur[i] = cur[i] + div_normal(cur[i] - prev, fac);
         }
       }
-      for (int i = 0; i < n; ++i) R[k][i] = cur[i];
+      for (int i = 0; i < n; ++i) R[k * D + i] = cur[i];
       if (k >= 1) {
         double err = 0.0;
         for (int i = 0; i < n; ++i) {
           const int b = i / 6, c = i % 6;
-          const double e = div_normal(fabs(cur[i] - R[k - 1][i]), sc[2 * b + (c < 3 ? 0 : 1)]);
+          const double e = div_normal(fabs(cur[i] - R[(k - 1) * D + i]), sc[2 * b + (c < 3 ? 0 : 1)]);
           err = e > err ? e : err;
         }
         if (err <= tol) {

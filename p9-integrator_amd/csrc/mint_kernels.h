@@ -270,10 +270,19 @@ __device__ __forceinline__ void momentum_lane(const double *v, int base, const L
 // changeover radius): each connected component, members in index order, is
 // drifted by one BS as one system, in place; *ingrp receives the bodies in a
 // group.  The component walk and the BS are those of the one-lane form.  Out
-// of line: taken only when some pair of the system is that close.
+// of line: taken only when some pair of the system is that close.  Its
+// arrays (a group's states and members, and the BS's) live in the caller's
+// workspace ws of massive_groups_ws<NM>() doubles (LDS in the ephemeris
+// kernel), so that no private memory is needed.
+template <int NM>
+constexpr int massive_groups_ws() {
+  return 7 * NM + bs_group_ws<NM>();  // sv (6 NM), the member indices (NM), the BS's
+}
+
 template <int NM>
 __device__ __noinline__ uint32_t massive_groups(double *S6, const uint32_t *adj, const double *__restrict__ prm,
-                                                StepConst sc, unsigned long long *stats, uint32_t *ingrp) {
+                                                StepConst sc, unsigned long long *stats, uint32_t *ingrp,
+                                                double *ws) {
   uint32_t st = ST_MASSIVE_ENC, in = 0u;
   unsigned int ng = 0, nb = 0, nr = 0;
   for (int r = 0; r < NM; ++r) {
@@ -287,14 +296,15 @@ __device__ __noinline__ uint32_t massive_groups(double *S6, const uint32_t *adj,
       comp |= nxt;
     }
     in |= comp;
-    int idx[NM], g = 0;
-    double sv[6 * NM];
+    double *sv = ws;
+    int *idx = (int *)(ws + 6 * NM);
+    int g = 0;
     for (int j = 0; j < NM; ++j) {
       if (!((comp >> j) & 1u)) continue;
       for (int c = 0; c < 6; ++c) sv[6 * g + c] = S6[6 * j + c];
       idx[g++] = j;
     }
-    st |= bs_drift_group<NM>(sv, g, idx, sc.h, prm, sc.mu, sc.eps2, sc.tol, nr);
+    st |= bs_drift_group<NM>(sv, g, idx, sc.h, prm, sc.mu, sc.eps2, sc.tol, nr, ws + 7 * NM);
     for (int q = 0; q < g; ++q)
       for (int c = 0; c < 6; ++c) S6[6 * idx[q] + c] = sv[6 * q + c];
     ++ng;
@@ -333,6 +343,7 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
   __shared__ double sh_state[SPW][NM][6];
   __shared__ uint32_t sh_adj[SPW][NM];
   __shared__ uint32_t sh_grp[SPW];
+  __shared__ double sh_ws[SPW][massive_groups_ws<NM>()];  // massive_groups' arrays, per system
   const int lane = (int)threadIdx.x;
   const int gs = lane / L, j = lane % L, base = gs * L;
   const int s_raw = s_begin + (int)blockIdx.x * SPW + gs;
@@ -414,7 +425,8 @@ __global__ void __launch_bounds__(64) ephemeris_kernel(int s_begin, int s_end, i
       __syncthreads();
       if (own && jb == 0) {
         uint32_t in = 0u;
-        if (sys_enc) st |= massive_groups<NM>(&sh_state[gs][0][0], &sh_adj[gs][0], prm, sc, stats, &in);
+        if (sys_enc)
+          st |= massive_groups<NM>(&sh_state[gs][0][0], &sh_adj[gs][0], prm, sc, stats, &in, &sh_ws[gs][0]);
         sh_grp[gs] = in;
       }
       __syncthreads();
