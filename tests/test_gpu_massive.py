@@ -1,7 +1,8 @@
 """Massive encounter groups and the many-body engine on the GPU, bit for bit
 against the oracle: the 1000-embryo configuration, a dense embryo ring with
 multi-body groups and physical radii, the lane-per-system engine and the
-many-body engine on the same 8-body set, a co-orbital pair beside test
+many-body engine on the same 8-body set, sixteen 8-body clusters eight to a
+wavefront, a co-orbital pair beside test
 particles, graph replay and call splitting, and checkpoint/resume."""
 import numpy as np
 import pytest
@@ -119,6 +120,25 @@ def test_lane_engine_and_many_body_engine_agree_on_8_bodies():
             _same(out, ref)
             assert np.array_equal(status, rst)
             assert stats["massive_groups"] == rstats["massive_groups"], (many, form)
+
+
+@pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_BATCHED])
+def test_lane_engine_groups_in_every_system_of_a_wave(form):
+    """Sixteen 8-body clusters in the lane-per-body engine: eight systems to a
+    wavefront, most of them inside a massive encounter group at the same
+    step, each running its group BS from its own workspace.  Every system
+    ends bit for bit where the oracle puts it."""
+    parts = [embryo_cluster(8, seed=10 + k) for k in range(16)]
+    st = mint.State(np.zeros(17, np.int64), np.concatenate([p.m for p in parts]),
+                    np.concatenate([p.mx for p in parts]), np.concatenate([p.mv for p in parts]),
+                    np.zeros((0, 3)), np.zeros((0, 3)))
+    ref, rst, rstats = O.run(st, 200, 0.002, bs_tol=1e-11)
+    assert rstats["massive_groups"] > 8 * 200 and rstats["massive_group_bodies"] > 2 * rstats["massive_groups"]
+    out, status, stats, _ = _gpu(st, 200, 0.002, bs_tol=1e-11, many_body=0, form=form)
+    _same(out, ref)
+    assert np.array_equal(status, rst)
+    for k in ("massive_groups", "massive_group_bodies", "overlaps"):
+        assert stats[k] == rstats[k], k
 
 
 @pytest.mark.parametrize("form", [mint.FORM_PLAIN, mint.FORM_FLAT, mint.FORM_BATCHED])
