@@ -2,8 +2,8 @@
 against the oracle: the 1000-embryo configuration, a dense embryo ring with
 multi-body groups and physical radii, the lane-per-system engine and the
 many-body engine on the same 8-body set, sixteen 8-body clusters eight to a
-wavefront, a co-orbital pair beside test
-particles, graph replay and call splitting, and checkpoint/resume."""
+wavefront, a co-orbital pair beside test particles, graph replay and call
+splitting, and checkpoint/resume."""
 import numpy as np
 import pytest
 
