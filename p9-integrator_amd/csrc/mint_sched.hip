@@ -183,10 +183,20 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
     const double *wv = w == 0 ? c->d_mv : c->d_snap[B] + nb;
     // ephemeris of window w (its end state is window w+1's snapshot, buffer
     // (w+1) mod 3, last read by the queue head of window w-2); enqueued first
-    // so that its few waves are placed before the queue head fills the CUs
-    if (w >= 2 && (rc = wait(sp, c->ev_tp, w - 2))) return rc;
-    if ((rc = window_eph(c, sp, C, wx, wv, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return rc;
-    if ((rc = record(c->ev_eph, w, sp))) return rc;
+    // so that its few waves are placed before the queue head fills the CUs,
+    // except in a call of one window: there the queue head, which the main
+    // kernel waits for, goes first (the graph starts its second root node
+    // 25-37 us after the first; 1.442 -> 1.434 and 1.459 -> 1.442 ms per
+    // 20-step call in same-process A/Bs, while calls of several windows were
+    // 0.2-0.5 % slower that way)
+    const bool head_first = W == 1;
+    auto eph = [&]() -> int {
+      int r2;
+      if (w >= 2 && (r2 = wait(sp, c->ev_tp, w - 2))) return r2;
+      if ((r2 = window_eph(c, sp, C, wx, wv, xs[B], rec[B], c->d_snap[(w + 1) % 3], kt))) return r2;
+      return record(c->ev_eph, w, sp);
+    };
+    if (!head_first && (rc = eph())) return rc;
     // queue head of window w, from the window-start state, beside the
     // window's ephemeris (window 0's queue was cleared by the previous call or
     // the upload)
@@ -200,6 +210,7 @@ int enqueue_batched(Ctx *c, const Shape &sh, int64_t nsteps, bool pipelined, KTi
                                         slist_head[p], stats),
                      "queue head launch")))
       return rc;
+    if (head_first && (rc = eph())) return rc;
     if ((rc = wait(sm, c->ev_eph, w))) return rc;
     // queue stream: tails of window w-1, then the head of window w
     if ((rc = record(c->ev_fork, w, sm)) || (rc = wait(sq, c->ev_fork, w))) return rc;
